@@ -1,0 +1,302 @@
+#!/usr/bin/env python3
+"""X-engine benchmark on one MI355X: correlated input GB/s, int8 MFMA % of peak, p99 dump latency.
+
+Workload (default): the 256-element x 128-channel array, 8192 samples per 20 Hz integration,
+config.h-format heaps (8192-byte payloads) resident in HBM. One step = one integration: corner-
+turn + lower-triangle int8 MFMA correlation + f32 integrate + grid/32x32 FFT image. Heaps are
+generated once by the seeded synthetic source (seed 0x4D495832) and replayed from two 512 MiB
+device rings, alternating, so no step re-reads the previous step's input from cache.
+
+Printed JSON line (rank 0): metric/value/unit as in the headline metric, plus
+  roofline       the dominant kernel's achieved algorithmic rate vs the MI355X peak
+  kernels        per-kernel average launch time and algorithmic rate
+  mfma_pct_peak  executed int8 MFMA ops / correlator time / 5.0 POP/s dense int8 peak
+  latency        p50/p99 arrival-to-dump (V + image complete, image on the host) with
+                 integrations arriving at 10x the 400 GbE-equivalent rate
+  plain_gpu      the per-channel-launch form (one corner-turn + one full-square GEMM per channel)
+  h2d            pinned host ring -> device header-strip copy rate (PCIe, not part of `value`)
+  cpu_baseline   the oracle's CPU correlator on this host's cores over a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "daqiri-mixer_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import mixer_amd as M  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0        # dense int8 MFMA, POP/s -> TOP/s
+GBE400_PAYLOAD_GBS = 50.0 * 8192 / 8256  # 400 GbE line rate carrying 8256-byte packets
+STAGES = ("corner_turn", "xcorr", "integrate", "image")
+
+CONFIGS = {
+    # name: (n_elem, grid_x, n_chan, k_integ, ng)
+    "cfg2": (256, 16, 128, 8192, 32),
+    "cfg1": (256, 16, 128, 1024, 32),
+    "cfg3": (1024, 32, 256, 4096, 64),
+}
+SEEDS = {"cfg1": 0x4D495831, "cfg2": 0x4D495832, "cfg3": 0x4D495833}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_cfg(name):
+    n, gx, c, k, ng = CONFIGS[name]
+    return M.default_cfg(n_elem=n, grid_x=gx, n_chan=c, k_integ=k, k_update=k, ng=ng,
+                         pkt_pitch=max(16384, 64 + 64 * c))
+
+
+def make_sky(name):
+    if name == "cfg3":  # Gaussian noise + 3 point sources, no planet
+        s = M.default_sky(sigma=0.5, planet_scale=0.0, star_amp=0.0)
+        s.n_src = 3
+        for i, (l, m, a) in enumerate([(0.0, 0.0, 1.0), (0.31, -0.12, 0.6), (-0.45, 0.27, 0.4)]):
+            s.src_l[i], s.src_m[i], s.src_amp[i] = l, m, a
+        return s
+    return M.default_sky()
+
+
+def synth_heaps(cfg, sky, seed, n_tb, seq0=0):
+    """Header-stripped heaps [n_tb][N][payload] from the synthetic source."""
+    N = cfg.n_elem
+    pk = M.synth_packets(cfg, sky, seed, seq0, n_tb * N, nthreads=min(16, os.cpu_count() or 1))
+    return np.ascontiguousarray(pk[:, cfg.hdr_bytes:]).reshape(n_tb, N, cfg.payload)
+
+
+def algorithmic(cfg):
+    N, C, K = cfg.n_elem, cfg.n_chan, cfg.k_integ
+    payload = N * C * K * 2
+    nb = N // 32
+    return {
+        "payload_bytes": payload,
+        "corner_turn_bytes": 2 * payload,                                  # read heaps + write tiles
+        "xcorr_ops": 8 * N * (N + 1) // 2 * K * C,                          # Hermitian complex MACs x 8
+        "xcorr_exec_ops": nb * (nb + 1) // 2 * 1024 * 4 * 2 * K * C,        # executed MFMA ops
+        "integrate_bytes": 2 * C * N * (N + 1) // 2 * 4 + 2 * C * N * N * 4,
+        "image_bytes": 2 * C * N * (N + 1) // 2 * 4 + C * cfg.ng * cfg.ng * 4,
+    }
+
+
+def cpu_baseline(cfg, heaps, n_tb, budget_s=15.0):
+    import pyoracle as O
+    threads = min(16, os.cpu_count() or 1)
+    C = cfg.n_chan
+    t0 = time.perf_counter()
+    O.xengine(heaps, n_tb, cfg.n_elem, C, 32, 0, 2, nthreads=threads)
+    dt = time.perf_counter() - t0
+    nch = int(max(2, min(C, budget_s / max(dt / 2, 1e-6))))
+    t0 = time.perf_counter()
+    O.xengine(heaps, n_tb, cfg.n_elem, C, 32, 0, nch, nthreads=threads)
+    dt = time.perf_counter() - t0
+    gbs = cfg.n_elem * nch * n_tb * 32 * 2 / dt / 1e9
+    return {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{nch} of {C} channels of one {n_tb * 32}-sample integration "
+                      f"({cfg.n_elem} elements), full-square reference math, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--latency-n", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} not supported)",
+              file=sys.stderr)
+        sys.exit(2)
+    assert M.device_count() >= 1, "no gfx950 device visible"
+    M.check(M.lib().mixer_set_device(0))
+
+    cfg = make_cfg(args.config)
+    alg = algorithmic(cfg)
+    N, C, K = cfg.n_elem, cfg.n_chan, cfg.k_integ
+    n_tb = K // 32
+    tb_bytes = N * cfg.payload
+    log(f"[bench] {args.config}: {N} el x {C} ch x {K} samples, {alg['payload_bytes'] / 2**20:.0f} MiB/integration")
+    t0 = time.perf_counter()
+    heaps = synth_heaps(cfg, make_sky(args.config), SEEDS[args.config], n_tb)
+    heaps2 = synth_heaps(cfg, make_sky(args.config), SEEDS[args.config], n_tb, seq0=n_tb * N)
+    log(f"[bench] synthetic heaps generated in {time.perf_counter() - t0:.1f} s")
+    rings = [M.DeviceBuffer.from_array(heaps), M.DeviceBuffer.from_array(heaps2)]
+
+    eng = M.Engine(cfg)
+    L = M.lib()
+    stream = L.mixer_compute_stream(eng.h)
+    for i in range(args.warmup):
+        eng.submit_device(rings[i % 2], n_tb)
+    eng.sync()
+
+    # ---- throughput: K steps bracketed by events on the compute stream ----
+    ev0, ev1 = M.C.c_void_p(), M.C.c_void_p()
+    M.check(L.mixer_event_create(M.C.byref(ev0)))
+    M.check(L.mixer_event_create(M.C.byref(ev1)))
+    eng.sync()
+    w0 = time.perf_counter()
+    M.check(L.mixer_event_record(ev0, stream))
+    for i in range(args.steps):
+        eng.submit_device(rings[i % 2], n_tb)
+    M.check(L.mixer_event_record(ev1, stream))
+    eng.sync()
+    wall = time.perf_counter() - w0
+    ms = M.C.c_float(0)
+    M.check(L.mixer_event_elapsed_ms(ev0, ev1, M.C.byref(ms)))
+    ms_total = float(ms.value)
+    ms_step = ms_total / args.steps
+    value = alg["payload_bytes"] * args.steps / (ms_total * 1e-3) / 1e9
+    log(f"[bench] {args.steps} steps: {ms_step:.4f} ms/step (events), wall {wall * 1e3 / args.steps:.4f} ms/step, "
+        f"{value:.1f} GB/s = {value / GBE400_PAYLOAD_GBS:.1f}x 400GbE-equivalent")
+
+    # ---- per-kernel timing (events around every launch, separate pass) ----
+    eng.set_timing(True)
+    for i in range(args.steps):
+        eng.submit_device(rings[i % 2], n_tb)
+    eng.sync()
+    kern = {}
+    for s, name in enumerate(STAGES):
+        tot, n = eng.timing(s)
+        kern[name] = {"avg_ms": tot / max(n, 1), "launches": n}
+    eng.set_timing(False)
+    kern["corner_turn"].update(bound="hbm", achieved=alg["corner_turn_bytes"] / (kern["corner_turn"]["avg_ms"] * 1e-3) / 1e9,
+                               unit="GB/s", algorithmic_per_launch=alg["corner_turn_bytes"])
+    xt = kern["xcorr"]["avg_ms"] * 1e-3
+    kern["xcorr"].update(bound="hbm", achieved=alg["payload_bytes"] / xt / 1e9, unit="GB/s",
+                         algorithmic_per_launch=alg["payload_bytes"],
+                         tops_algorithmic=alg["xcorr_ops"] / xt / 1e12, tops_executed=alg["xcorr_exec_ops"] / xt / 1e12)
+    kern["integrate"].update(bound="hbm", achieved=alg["integrate_bytes"] / (kern["integrate"]["avg_ms"] * 1e-3) / 1e9,
+                             unit="GB/s", algorithmic_per_launch=alg["integrate_bytes"])
+    kern["image"].update(bound="hbm", achieved=alg["image_bytes"] / (kern["image"]["avg_ms"] * 1e-3) / 1e9,
+                         unit="GB/s", algorithmic_per_launch=alg["image_bytes"])
+    for k in kern.values():
+        for f in ("avg_ms", "achieved", "tops_algorithmic", "tops_executed"):
+            if f in k:
+                k[f] = round(k[f], 4)
+    dom = max(STAGES, key=lambda s: kern[s]["avg_ms"])
+    mfma_pct = 100.0 * alg["xcorr_exec_ops"] / xt / 1e12 / INT8_PEAK_TOPS
+    log(f"[bench] kernels: " + ", ".join(f"{s} {kern[s]['avg_ms']:.4f} ms" for s in STAGES) +
+        f"; MFMA {mfma_pct:.1f}% of peak")
+
+    # ---- latency: integrations arriving at 10x the 400 GbE-equivalent rate ----
+    period = alg["payload_bytes"] / (10 * GBE400_PAYLOAD_GBS * 1e9)
+    img = np.empty((C, cfg.ng, cfg.ng), np.float32)
+    vmax = np.empty(C, np.float32)
+    vmin = np.empty(C, np.float32)
+    idx = M.C.c_uint64(0)
+    lat = []
+    t_next = time.perf_counter()
+    for i in range(args.latency_n):
+        while time.perf_counter() < t_next:
+            pass
+        t_arr = time.perf_counter()
+        eng.submit_device(rings[i % 2], n_tb)
+        M.check(L.mixer_read_image(eng.h, M.ptr(img), M.ptr(vmax), M.ptr(vmin), M.C.byref(idx)))
+        lat.append((time.perf_counter() - t_arr) * 1e3)
+        t_next = max(t_next + period, time.perf_counter())
+    lat = np.array(lat)
+    latency = {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+               "n": len(lat), "arrival_period_ms": round(period * 1e3, 4),
+               "definition": "last heap of an integration submitted -> its V cube and image complete in HBM "
+                             "and the image cube copied to the host"}
+    log(f"[bench] latency p50 {latency['p50_ms']:.3f} ms p99 {latency['p99_ms']:.3f} ms over {len(lat)} integrations")
+    eng.close()
+
+    # ---- plain per-channel-launch GPU form ----
+    plain = None
+    if not args.no_plain:
+        d_t = M.DeviceBuffer(M.tiles_bytes(cfg, n_tb))
+        d_r = M.DeviceBuffer(C * N * N * 4)
+        d_i = M.DeviceBuffer(C * N * N * 4)
+        s = M.C.c_void_p()
+        M.check(L.mixer_stream_create(M.C.byref(s)))
+        M.xcorr_plain(cfg, rings[0], n_tb, d_t, d_r, d_i, stream=s)
+        M.check(L.mixer_stream_sync(s))
+        reps = 3
+        M.check(L.mixer_event_record(ev0, s))
+        for i in range(reps):
+            M.xcorr_plain(cfg, rings[i % 2], n_tb, d_t, d_r, d_i, stream=s)
+        M.check(L.mixer_event_record(ev1, s))
+        M.check(L.mixer_event_elapsed_ms(ev0, ev1, M.C.byref(ms)))
+        pms = ms.value / reps
+        plain = {"value": round(alg["payload_bytes"] / (pms * 1e-3) / 1e9, 3), "unit": "GB/s",
+                 "ms_per_integration": round(pms, 4),
+                 "form": "per channel: one corner-turn launch + one full-square GEMM launch (no integrate/image)"}
+        log(f"[bench] plain per-channel form: {pms:.3f} ms/integration, {plain['value']:.1f} GB/s")
+        M.check(L.mixer_stream_destroy(s))
+
+    # ---- H2D from a pinned host ring (PCIe; reported separately) ----
+    h2d = None
+    try:
+        nblk = 64
+        pitch = int(cfg.pkt_pitch)
+        hp = M.C.c_void_p()
+        M.check(L.mixer_host_alloc(nblk * N * pitch, M.C.byref(hp)))
+        ring = np.ctypeslib.as_array((M.C.c_uint8 * (nblk * N * pitch)).from_address(hp.value)).reshape(nblk * N, pitch)
+        ring[:, cfg.hdr_bytes:cfg.hdr_bytes + cfg.payload] = heaps[:nblk].reshape(nblk * N, cfg.payload)
+        for k in range(nblk * N):
+            ring[k, :cfg.hdr_bytes] = 0
+            ring[k, cfg.seq_byte:cfg.seq_byte + 4] = np.frombuffer(int(k).to_bytes(4, "big"), np.uint8)
+        c2 = make_cfg(args.config)
+        c2.flags = M.CFG_NOFLOP
+        e2 = M.Engine(c2)
+        e2.sync()
+        reps = 8
+        t0 = time.perf_counter()
+        for r in range(reps):
+            e2.token_wait(e2.submit_heaps(ring, pitch, nblk * N))
+        e2.sync()
+        dt = time.perf_counter() - t0
+        h2d = {"value": round(nblk * tb_bytes * reps / dt / 1e9, 3), "unit": "GB/s",
+               "path": "pinned host ring (pitch 16384) -> hipMemcpy2DAsync header strip"}
+        e2.close()
+        M.check(L.mixer_host_free(hp))
+        log(f"[bench] H2D {h2d['value']:.1f} GB/s")
+    except Exception as ex:  # reported, never fatal
+        log(f"[bench] H2D measurement skipped: {ex}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, heaps, n_tb)
+        log(f"[bench] CPU correlator {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['sample']})")
+
+    d = kern[dom]
+    peak = HBM_PEAK_GBS
+    out = {
+        "metric": "correlated input GB/s + int8 MFMA % peak, p99 dump latency; 256-elt×128-ch",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {N} elements x {C} channels, {K} samples per integration, "
+                               f"config.h heaps resident in HBM; corner-turn + int8 MFMA GEMM + f32 integrate + "
+                               f"{cfg.ng}x{cfg.ng} FFT image per step",
+                   "elements": N, "channels": C, "samples_per_integration": K,
+                   "payload_bytes_per_step": alg["payload_bytes"]},
+        "x_400gbe_equivalent": round(value / GBE400_PAYLOAD_GBS, 2),
+        "mfma_pct_peak": round(mfma_pct, 2),
+        "roofline": {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"], "peak": peak,
+                     "unit": "GB/s", "frac": round(d["achieved"] / peak, 4), "traffic": None},
+        "kernels": kern,
+        "latency": latency,
+        "plain_gpu": plain,
+        "h2d": h2d,
+        "cpu_baseline": cpu,
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,27 @@
+// internal.h -- shared helpers of the host side of libmixer (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "kernels.h"
+#include "mixer.h"
+
+namespace mixer {
+
+int hip_status(hipError_t e);
+int plan_slots(const mixer_array_cfg& c);
+std::vector<std::pair<int, int>> sblocks_for(int nb);
+int get_plan(const mixer_array_cfg& c, XcorrPlan* p);
+int get_twiddles(int ng, const float2** out);
+int current_device();
+
+#define MIXER_TRY(x)                  \
+    do {                              \
+        int st_ = (x);                \
+        if (st_ != MIXER_OK) return st_; \
+    } while (0)
+#define MIXER_HIP(x) MIXER_TRY(::mixer::hip_status(x))
+
+}  // namespace mixer

@@ -1,0 +1,123 @@
+// synth.cpp -- seeded synthetic F-engine packets in the config.h wire format. Replaces the
+// DAQIRI capture (out of scope) and restates the transmitter's signal model, k_sky
+// (src/tx_host.cu:41-60), plus Gaussian noise and extra point sources. Header and seq baking
+// follow src/tx_host.cu:136-143 (64-byte header, seq big-endian at byte 48, element = seq % N).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "mixer.h"
+
+namespace {
+
+// splitmix64 finaliser: a counter-based generator keyed by (seed, element, channel, sample).
+inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Two independent N(0,1) by Box-Muller from one 64-bit key.
+inline void gauss2(uint64_t key, double* a, double* b) {
+    const uint64_t r1 = mix64(key), r2 = mix64(key ^ 0xD1B54A32D192ED03ull);
+    const double u1 = ((double)(r1 >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+    const double u2 = (double)(r2 >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    *a = rad * std::cos(2.0 * M_PI * u2);
+    *b = rad * std::sin(2.0 * M_PI * u2);
+}
+
+inline int8_t q8(double v) {
+    long q = std::lrint(v);
+    return (int8_t)(q < -127 ? -127 : (q > 127 ? 127 : q));
+}
+
+void write_header(uint8_t* p, const mixer_array_cfg& c, uint32_t seq) {
+    memset(p, 0, (size_t)c.hdr_bytes);
+    p[c.seq_byte + 0] = (uint8_t)(seq >> 24);
+    p[c.seq_byte + 1] = (uint8_t)(seq >> 16);
+    p[c.seq_byte + 2] = (uint8_t)(seq >> 8);
+    p[c.seq_byte + 3] = (uint8_t)seq;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mixer_default_sky(mixer_sky* s) {
+    if (!s) return;
+    memset(s, 0, sizeof(*s));
+    s->planet_l = 0.525;   // rho = 0.525 at orbit phase 0 (src/tx_host.cu:86,106)
+    s->planet_m = 0.0;
+    s->sigma = 0.5;
+    s->star_amp = 2.0;     // Astar (src/tx_host.cu:42)
+    s->planet_scale = 0.71;  // src/tx_host.cu:48
+    s->nch_gen = 256;      // MIXER_NCH (src/config.h:28)
+    s->n_src = 0;
+}
+
+int mixer_synth_packets(const mixer_array_cfg* cfg, const mixer_sky* sky, uint64_t seed,
+                        uint64_t seq0, int npkts, uint8_t* out, size_t pitch, int nthreads) {
+    if (!cfg || !sky || !out || npkts < 0) return MIXER_ERR_ARG;
+    int st = mixer_validate_cfg(cfg);
+    if (st) return st;
+    if (pitch < mixer_wire_bytes(cfg) || sky->n_src < 0 || sky->n_src > 4 || sky->nch_gen < 2)
+        return MIXER_ERR_ARG;
+    const mixer_array_cfg c = *cfg;
+    const int N = c.n_elem, C = c.n_chan, T = c.t_pkt;
+    // per (element) phase terms and per (channel) planet amplitude, in double
+    std::vector<double> pre(N), pim(N), apl(C);
+    std::vector<double> sre((size_t)N, 0.0), sim((size_t)N, 0.0);
+    for (int a = 0; a < N; ++a) {
+        const int px = a % c.grid_x, qy = a / c.grid_x;
+        const double phi = M_PI * (px * sky->planet_l + qy * sky->planet_m);
+        pre[a] = std::cos(phi);
+        pim[a] = std::sin(phi);
+        for (int k = 0; k < sky->n_src; ++k) {
+            const double ph = M_PI * (px * sky->src_l[k] + qy * sky->src_m[k]);
+            sre[a] += sky->src_amp[k] * std::cos(ph);
+            sim[a] += sky->src_amp[k] * std::sin(ph);
+        }
+    }
+    for (int ch = 0; ch < C; ++ch)
+        apl[ch] = sky->planet_scale * sky->star_amp * std::sqrt(2.0 * ch / (double)(sky->nch_gen - 1));
+    const double sig = sky->sigma;
+    auto work = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const uint64_t seq = seq0 + (uint64_t)i;
+            const int a = (int)(seq % (uint64_t)N);
+            const uint64_t tb = seq / (uint64_t)N;
+            uint8_t* p = out + (size_t)i * pitch;
+            write_header(p, c, (uint32_t)seq);
+            int8_t* re = (int8_t*)(p + c.hdr_bytes);
+            int8_t* im = re + (size_t)C * T;
+            for (int ch = 0; ch < C; ++ch) {
+                const double mr = sky->star_amp + apl[ch] * pre[a] + sre[a];
+                const double mi = apl[ch] * pim[a] + sim[a];
+                const uint64_t kac = mix64(mix64(seed ^ (uint64_t)a) ^ ((uint64_t)ch << 32));
+                for (int t = 0; t < T; ++t) {
+                    double nr = 0, ni = 0;
+                    if (sig > 0) gauss2(mix64(kac ^ (tb * (uint64_t)T + (uint64_t)t)), &nr, &ni);
+                    re[ch * T + t] = q8(25.0 * (mr + sig * nr));
+                    im[ch * T + t] = q8(25.0 * (mi + sig * ni));
+                }
+            }
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 64) nt = 64;
+    if (npkts < 64 * nt) nt = 1 + npkts / 64;
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; ++k) {
+        const int i0 = (int)((long)npkts * k / nt), i1 = (int)((long)npkts * (k + 1) / nt);
+        th.emplace_back(work, i0, i1);
+    }
+    for (auto& t : th) t.join();
+    return MIXER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+"""ctypes binding of libmixer (include/mixer.h), the gfx950 X-engine.
+
+The binding mirrors the reference's operator surface (src/rx_host_corr.cu: k_packAB, cgemm_blk_t,
+k_flush_cube, k_grid_cube + cuFFT, the --dump record and the Corr32 image cube). It loads the
+in-tree library daqiri-mixer_amd/lib/libmixer.so and raises if it is missing: there is no CPU
+fallback. Device memory, streams and events come from the library's own plumbing entry points,
+so exactly one HIP runtime is loaded in the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmixer.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BUDGET, ERR_NODEV, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
+CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP = 1, 2, 4
+H2D, D2H, D2D = 0, 1, 2
+
+
+class MixerError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+
+
+class ArrayCfg(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int32), ("grid_x", C.c_int32), ("n_chan", C.c_int32), ("t_pkt", C.c_int32),
+        ("hdr_bytes", C.c_int32), ("seq_byte", C.c_int32), ("k_integ", C.c_int32),
+        ("k_update", C.c_int32), ("ng", C.c_int32), ("flags", C.c_int32), ("pkt_pitch", C.c_int64),
+    ]
+
+    @property
+    def payload(self) -> int:
+        return 2 * self.n_chan * self.t_pkt
+
+    @property
+    def wire(self) -> int:
+        return self.hdr_bytes + self.payload
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("heaps_in", "time_blocks", "vupdates", "integrations",
+                                           "imissed", "misalign", "seq_gaps", "wrap128")] + [
+        ("last_dump_latency_ms", C.c_double)]
+
+
+class Sky(C.Structure):
+    _fields_ = [("planet_l", C.c_double), ("planet_m", C.c_double), ("sigma", C.c_double),
+                ("star_amp", C.c_double), ("planet_scale", C.c_double), ("nch_gen", C.c_int32),
+                ("n_src", C.c_int32), ("src_l", C.c_double * 4), ("src_m", C.c_double * 4),
+                ("src_amp", C.c_double * 4)]
+
+
+_lib = None
+P, I32, SZ, U64 = C.c_void_p, C.c_int, C.c_size_t, C.c_uint64
+
+# (name, restype, argtypes) for every entry point in include/mixer.h
+SIGNATURES = [
+    ("mixer_default_cfg", None, [P]),
+    ("mixer_validate_cfg", I32, [P]),
+    ("mixer_payload_bytes", SZ, [P]),
+    ("mixer_wire_bytes", SZ, [P]),
+    ("mixer_version", C.c_char_p, []),
+    ("mixer_strerror", C.c_char_p, [I32]),
+    ("mixer_device_count", I32, [P]),
+    ("mixer_tiles_bytes", SZ, [P, I32]),
+    ("mixer_corner_turn", I32, [P, P, P, I32, P, P]),
+    ("mixer_acc_bytes", SZ, [P]),
+    ("mixer_acc_slots", I32, [P]),
+    ("mixer_xcorr", I32, [P, P, P, I32, P, I32]),
+    ("mixer_integrate", I32, [P, P, P, P, I32]),
+    ("mixer_image", I32, [P, P, P, P, P, P]),
+    ("mixer_xcorr_plain", I32, [P, P, P, I32, P, P, P]),
+    ("mixer_create", I32, [P, I32, P]),
+    ("mixer_destroy", I32, [P]),
+    ("mixer_get_cfg", I32, [P, P]),
+    ("mixer_submit_heaps", I32, [P, P, SZ, I32, U64, P]),
+    ("mixer_submit_device", I32, [P, P, I32, U64, P]),
+    ("mixer_token_done", I32, [P, U64]),
+    ("mixer_token_wait", I32, [P, U64]),
+    ("mixer_sync", I32, [P]),
+    ("mixer_dump", I32, [P, P, I32, P]),
+    ("mixer_read_image", I32, [P, P, P, P, P]),
+    ("mixer_corr32_bytes", SZ, [P]),
+    ("mixer_corr32_init", I32, [P, P]),
+    ("mixer_publish_image", I32, [P, P, SZ]),
+    ("mixer_get_stats", I32, [P, P]),
+    ("mixer_set_timing", I32, [P, I32]),
+    ("mixer_get_timing", I32, [P, I32, P, P]),
+    ("mixer_compute_stream", P, [P]),
+    ("mixer_set_device", I32, [I32]),
+    ("mixer_dev_alloc", I32, [SZ, P]),
+    ("mixer_dev_free", I32, [P]),
+    ("mixer_host_alloc", I32, [SZ, P]),
+    ("mixer_host_free", I32, [P]),
+    ("mixer_memcpy", I32, [P, P, SZ, I32, P]),
+    ("mixer_memset", I32, [P, I32, SZ, P]),
+    ("mixer_stream_create", I32, [P]),
+    ("mixer_stream_destroy", I32, [P]),
+    ("mixer_stream_sync", I32, [P]),
+    ("mixer_event_create", I32, [P]),
+    ("mixer_event_destroy", I32, [P]),
+    ("mixer_event_record", I32, [P, P]),
+    ("mixer_event_elapsed_ms", I32, [P, P, P]),
+    ("mixer_device_sync", I32, []),
+    ("mixer_default_sky", None, [P]),
+    ("mixer_synth_packets", I32, [P, P, U64, U64, I32, P, SZ, I32]),
+]
+
+
+def lib():
+    """Load libmixer.so (raises if it has not been built: the HIP path has no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libmixer is not built ({LIB_PATH} missing): run `make -C daqiri-mixer_amd`")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def strerror(status: int) -> str:
+    return lib().mixer_strerror(status).decode()
+
+
+def check(status: int, what: str = "libmixer"):
+    if status != OK:
+        raise MixerError(status, what)
+    return status
+
+
+def ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_cfg(**kw) -> ArrayCfg:
+    c = ArrayCfg()
+    lib().mixer_default_cfg(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def default_sky(**kw) -> Sky:
+    s = Sky()
+    lib().mixer_default_sky(C.byref(s))
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().mixer_device_count(C.byref(n)))
+    return n.value
+
+
+def synth_packets(cfg: ArrayCfg, sky: Sky, seed: int, seq0: int, npkts: int, pitch: int | None = None,
+                  nthreads: int = 0) -> np.ndarray:
+    """Host packets [npkts][pitch] in the wire format (header + planar payload)."""
+    pitch = cfg.wire if pitch is None else pitch
+    out = np.zeros((npkts, pitch), np.uint8)
+    check(lib().mixer_synth_packets(C.byref(cfg), C.byref(sky), seed, seq0, npkts, ptr(out), pitch, nthreads),
+          "mixer_synth_packets")
+    return out
+
+
+class DeviceBuffer:
+    """Device memory owned by libmixer's HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().mixer_dev_alloc(max(self.nbytes, 1), C.byref(p)), "mixer_dev_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().mixer_memcpy(self.ptr, ptr(a), a.nbytes, H2D, stream), "mixer_memcpy")
+        check(lib().mixer_stream_sync(stream), "mixer_stream_sync")
+
+    def download(self, dtype, shape, stream=None) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().mixer_stream_sync(stream), "mixer_stream_sync")
+        check(lib().mixer_memcpy(ptr(out), self.ptr, out.nbytes, D2H, stream), "mixer_memcpy")
+        check(lib().mixer_stream_sync(stream), "mixer_stream_sync")
+        return out
+
+    def zero(self, stream=None):
+        check(lib().mixer_memset(self.ptr, 0, self.nbytes, stream), "mixer_memset")
+
+    def free(self):
+        if self.ptr:
+            check(lib().mixer_dev_free(self.ptr), "mixer_dev_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.mixer_dev_free(self.ptr)
+        except Exception:
+            pass
+
+
+# ---- operators on device buffers (the kernel seam) ------------------------------------------
+def corner_turn(cfg, d_heaps: DeviceBuffer, n_tb: int, d_tiles: DeviceBuffer, d_wrap=None, stream=None):
+    check(lib().mixer_corner_turn(stream, C.byref(cfg), d_heaps.ptr, n_tb, d_tiles.ptr,
+                                  d_wrap.ptr if d_wrap else None), "mixer_corner_turn")
+
+
+def xcorr(cfg, d_tiles: DeviceBuffer, n_tb: int, d_acc: DeviceBuffer, accumulate=False, stream=None):
+    check(lib().mixer_xcorr(stream, C.byref(cfg), d_tiles.ptr, n_tb, d_acc.ptr, 1 if accumulate else 0),
+          "mixer_xcorr")
+
+
+def integrate(cfg, d_acc: DeviceBuffer, d_v: DeviceBuffer, accumulate=False, stream=None):
+    check(lib().mixer_integrate(stream, C.byref(cfg), d_acc.ptr, d_v.ptr, 1 if accumulate else 0),
+          "mixer_integrate")
+
+
+def image(cfg, d_acc: DeviceBuffer, d_img: DeviceBuffer, d_vmax: DeviceBuffer, d_vmin: DeviceBuffer, stream=None):
+    check(lib().mixer_image(stream, C.byref(cfg), d_acc.ptr, d_img.ptr, d_vmax.ptr, d_vmin.ptr), "mixer_image")
+
+
+def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):
+    check(lib().mixer_xcorr_plain(stream, C.byref(cfg), d_heaps.ptr, n_tb, d_tiles.ptr, d_vre.ptr, d_vim.ptr),
+          "mixer_xcorr_plain")
+
+
+def tiles_bytes(cfg, n_tb: int) -> int:
+    return int(lib().mixer_tiles_bytes(C.byref(cfg), n_tb))
+
+
+def acc_bytes(cfg) -> int:
+    return int(lib().mixer_acc_bytes(C.byref(cfg)))
+
+
+def acc_slots(cfg) -> int:
+    return int(lib().mixer_acc_slots(C.byref(cfg)))
+
+
+def unpack_acc(acc: np.ndarray, cfg) -> tuple[np.ndarray, np.ndarray]:
+    """Tile-packed int32 accumulator [slots][C][tiles][2][32][32] -> full-square (Vre, Vim)
+    [C][N][N] in int64: slots summed, upper triangle = conj(lower). Host-side view for tests."""
+    N, Cc = cfg.n_elem, cfg.n_chan
+    nb = N // 32
+    a = acc.reshape(-1, Cc, nb * (nb + 1) // 2, 2, 32, 32).astype(np.int64).sum(axis=0)
+    vre = np.zeros((Cc, N, N), np.int64)
+    vim = np.zeros((Cc, N, N), np.int64)
+    t = 0
+    for I in range(nb):
+        for J in range(I + 1):
+            vre[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32] = a[:, t, 0]
+            vim[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32] = a[:, t, 1]
+            if I != J:
+                vre[:, J * 32:(J + 1) * 32, I * 32:(I + 1) * 32] = a[:, t, 0].transpose(0, 2, 1)
+                vim[:, J * 32:(J + 1) * 32, I * 32:(I + 1) * 32] = -a[:, t, 1].transpose(0, 2, 1)
+            t += 1
+    return vre, vim
+
+
+# ---- engine context ---------------------------------------------------------------------------
+class Engine:
+    """The X-engine context (the reference's gpu_thread + publish loop)."""
+
+    def __init__(self, cfg: ArrayCfg, device: int = 0):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().mixer_create(C.byref(cfg), device, C.byref(h)), "mixer_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(lib().mixer_destroy(self.h), "mixer_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit_heaps(self, pkts: np.ndarray, pitch: int, npkts: int, arrival_ns: int = 0) -> int:
+        tok = C.c_uint64(0)
+        check(lib().mixer_submit_heaps(self.h, ptr(pkts), pitch, npkts, arrival_ns, C.byref(tok)),
+              "mixer_submit_heaps")
+        return tok.value
+
+    def submit_device(self, d: DeviceBuffer, n_tb: int, arrival_ns: int = 0, offset: int = 0) -> int:
+        tok = C.c_uint64(0)
+        p = C.c_void_p(d.ptr.value + offset)
+        check(lib().mixer_submit_device(self.h, p, n_tb, arrival_ns, C.byref(tok)), "mixer_submit_device")
+        return tok.value
+
+    def token_wait(self, tok: int):
+        check(lib().mixer_token_wait(self.h, tok), "mixer_token_wait")
+
+    def sync(self):
+        check(lib().mixer_sync(self.h), "mixer_sync")
+
+    def dump(self, cumulative=False):
+        c = self.cfg
+        out = np.empty((c.n_chan, 2, c.n_elem, c.n_elem), np.float32)
+        idx = C.c_uint64(0)
+        check(lib().mixer_dump(self.h, ptr(out), 1 if cumulative else 0, C.byref(idx)), "mixer_dump")
+        return out, idx.value
+
+    def read_image(self):
+        c = self.cfg
+        img = np.empty((c.n_chan, c.ng, c.ng), np.float32)
+        vmax = np.empty(c.n_chan, np.float32)
+        vmin = np.empty(c.n_chan, np.float32)
+        idx = C.c_uint64(0)
+        check(lib().mixer_read_image(self.h, ptr(img), ptr(vmax), ptr(vmin), C.byref(idx)), "mixer_read_image")
+        return img, vmax, vmin, idx.value
+
+    def publish(self, region: np.ndarray):
+        check(lib().mixer_publish_image(self.h, ptr(region), region.nbytes), "mixer_publish_image")
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().mixer_get_stats(self.h, C.byref(s)), "mixer_get_stats")
+        return s
+
+    def set_timing(self, on: bool):
+        check(lib().mixer_set_timing(self.h, 1 if on else 0), "mixer_set_timing")
+
+    def timing(self, stage: int):
+        ms = C.c_double(0)
+        n = C.c_uint64(0)
+        check(lib().mixer_get_timing(self.h, stage, C.byref(ms), C.byref(n)), "mixer_get_timing")
+        return ms.value, n.value
+
+
+def corr32_bytes(cfg) -> int:
+    return int(lib().mixer_corr32_bytes(C.byref(cfg)))

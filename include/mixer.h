@@ -1,0 +1,246 @@
+/* mixer.h -- C ABI of libmixer, the MI355X (gfx950) X-engine.
+ *
+ * The reference (DAQIRI-MIXER) is one executable with no library API; its operator surface is
+ * four seams, each replaced here by extern "C" entry points with plain pointers and sizes:
+ *   input seam   Slot{mr_ptr} of 256 packets at pitch MR_STRIDE   src/rx_host_corr.cu:120-121,136-141,159
+ *   kernel seam  k_packAB(bb, Are, Aim, Bre, Bim)                  src/rx_host_corr.cu:61-72
+ *                cgemm_blk_t<ACC>(M,N,K,Are,Aim,Bre,Bim,Cre,Cim)  src/cgemm.cuh:84-125
+ *                k_flush_cube(Vcube, Vire, Viim)                   src/rx_host_corr.cu:90-95
+ *                k_diff / k_grid_cube / cufftExecC2C               src/rx_host_corr.cu:74-85,205-206,222
+ *   output seam  --dump record [C][2][N][N] f32                    src/rx_host_corr.cu:187,210-227
+ *                Corr32 in /dev/shm/corr32                          src/rx_host_corr.cu:97-105,229-236
+ *   CLI          rx_host_corr <cfg> [--seconds][--fps][--device][--noflop][--flush][--dump]
+ *                                                                   src/rx_host_corr.cu:177-183
+ *
+ * Error convention: every int-returning call returns MIXER_OK (0) or a negative mixer status;
+ * the reference instead exits the process on any CUDA error (CK -> exit(1),
+ * src/rx_host_corr.cu:35). The CLI maps a nonzero status to exit code 1.
+ *
+ * Threading: one submitting thread per context (the reference's gpu_thread role); mixer_dump,
+ * mixer_read_image and mixer_get_stats may be called from one other thread (its main/pub role).
+ *
+ * Value domain: inputs are complex int8 with components in [-127, 127], which the reference's
+ * own transmitter guarantees (clip at src/tx_host.cu:54-55). Inside that domain V is exactly
+ * Hermitian and every output is bit-identical to the reference's full-square product. An
+ * imaginary sample of -128 is counted (mixer_stats.wrap128): the lower triangle and the
+ * diagonal tiles stay bit-identical to the reference even then, but the mirrored upper
+ * triangle is conj(lower), which the reference's wrapped negation does not reproduce.
+ */
+#ifndef MIXER_H
+#define MIXER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+    MIXER_OK = 0,
+    MIXER_ERR_ARG = -1,      /* bad argument or unsupported geometry */
+    MIXER_ERR_HIP = -2,      /* a HIP runtime call failed */
+    MIXER_ERR_NOMEM = -3,    /* device or pinned host allocation failed */
+    MIXER_ERR_STATE = -4,    /* call not valid in the context's current state */
+    MIXER_ERR_BUDGET = -5,   /* HBM budget would be exceeded (serving admission) */
+    MIXER_ERR_NODEV = -6,    /* no usable gfx950 device */
+    MIXER_ERR_OVERFLOW = -7, /* samples per integration exceed the exact int32 range */
+};
+
+/* ---- array geometry and wire format -------------------------------------------------------
+ * Defaults (mixer_default_cfg) are the reference's compile-time constants:
+ * MIXER_NANT=256, MIXER_GRID=16, MIXER_ACTIVE_NCH=128, MIXER_TPKT=32, MIXER_HDR_BYTES=64,
+ * MIXER_SEQ_BYTE=48 (src/config.h:26-38); KPCH=1024 per V-update, MR_STRIDE=16384, NG=32
+ * (src/rx_host_corr.cu:40-51); 8192 samples per 20 Hz integration.
+ * Packet: hdr_bytes header (seq big-endian u32 at seq_byte), then a payload of
+ * 2*n_chan*t_pkt bytes, PLANAR: re[c*t_pkt + t] for all c, then im[...] (src/config.h:10-23).
+ * Element of a packet = seq % n_elem. n_elem consecutive packets (seq % n_elem == 0 first)
+ * form one time block of t_pkt samples. */
+typedef struct mixer_array_cfg {
+    int32_t n_elem;    /* elements N; a multiple of 32, 32..1024 */
+    int32_t grid_x;    /* elements per grid row; element a sits at (a % grid_x, a / grid_x) */
+    int32_t n_chan;    /* channels per packet C, 1..1024 */
+    int32_t t_pkt;     /* samples per packet per channel; must be 32 (one MFMA K step) */
+    int32_t hdr_bytes; /* header bytes before the payload (64) */
+    int32_t seq_byte;  /* byte offset of the big-endian u32 seq (48) */
+    int32_t k_integ;   /* samples per integration (dump); multiple of k_update, <= 65536 */
+    int32_t k_update;  /* samples per V-update (one corner-turn + GEMM launch); multiple of t_pkt */
+    int32_t ng;        /* image grid side (32 or 64); >= 2*grid_x and >= 2*(n_elem/grid_x) */
+    int32_t flags;     /* MIXER_CFG_* bits */
+    int64_t pkt_pitch; /* host ring row pitch in bytes (16384) */
+} mixer_array_cfg;
+
+#define MIXER_CFG_CUMULATIVE 1 /* also keep the reference's cumulative-since-start f32 cube */
+#define MIXER_CFG_NO_IMAGE 2   /* skip gridding + FFT at each integration */
+#define MIXER_CFG_NOFLOP 4     /* ingest only: copy heaps, skip all compute (--noflop,
+                                  src/rx_host_corr.cu:161) */
+
+void mixer_default_cfg(mixer_array_cfg* cfg);
+int mixer_validate_cfg(const mixer_array_cfg* cfg);
+size_t mixer_payload_bytes(const mixer_array_cfg* cfg); /* 2*n_chan*t_pkt */
+size_t mixer_wire_bytes(const mixer_array_cfg* cfg);    /* hdr_bytes + payload */
+
+const char* mixer_version(void);
+const char* mixer_strerror(int status);
+int mixer_device_count(int* n); /* visible gfx950 devices */
+
+/* ---- operator entry points on device pointers ---------------------------------------------
+ * stream is a hipStream_t (NULL = the default stream). Shapes come from cfg; n_tb is the
+ * number of time blocks (each t_pkt samples) in the call.
+ *
+ * Corner-turn: replaces k_packAB (src/rx_host_corr.cu:56-72).
+ *   d_heaps  header-stripped payloads [n_tb][n_elem][payload] (the reference's g_bb layout)
+ *   d_tiles  int8 element x time tiles [n_chan][n_tb][2][n_elem][32]: for channel c and time
+ *            block tb, a re tile [n_elem][32] followed by an im tile [n_elem][32]
+ *            (mixer_tiles_bytes). The reference's B = conj(X) is derived inside the GEMM.
+ *   d_wrap   optional u32 counter (may be NULL): incremented once per workgroup that saw an
+ *            imaginary sample of -128. */
+size_t mixer_tiles_bytes(const mixer_array_cfg* cfg, int n_tb);
+int mixer_corner_turn(void* stream, const mixer_array_cfg* cfg, const uint8_t* d_heaps, int n_tb,
+                      int8_t* d_tiles, uint32_t* d_wrap);
+
+/* Correlate: replaces cgemm_blk_t<ACC> (src/cgemm.cuh:84-125) on the lower triangle.
+ *   d_acc    int32 accumulator, mixer_acc_bytes(cfg) bytes: [n_slots][n_chan][n_tiles][2][32][32]
+ *            with tile t = I*(I+1)/2 + J for 32x32 element blocks I >= J, plane 0 = Re, 1 = Im,
+ *            row = element of block I, column = element of block J, and n_slots partial sums
+ *            (split over time) that mixer_integrate adds. accumulate=0 overwrites (ACC=false),
+ *            1 adds into it (ACC=true). Diagonal tiles hold the full 32x32 block. */
+size_t mixer_acc_bytes(const mixer_array_cfg* cfg);
+int mixer_acc_slots(const mixer_array_cfg* cfg);
+int mixer_xcorr(void* stream, const mixer_array_cfg* cfg, const int8_t* d_tiles, int n_tb,
+                int32_t* d_acc, int accumulate);
+
+/* Integrate: replaces k_flush_cube (src/rx_host_corr.cu:86-95). Sums the accumulator slots in
+ * int32 (exact), converts once to f32, and writes the full square
+ *   d_v [n_chan][2][n_elem][n_elem] f32 (the --dump record layout), upper triangle = conj(lower).
+ * accumulate=0 stores V; 1 adds V into d_v (the reference's cumulative cube). */
+int mixer_integrate(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_v,
+                    int accumulate);
+
+/* Image: replaces k_diff + k_grid_cube + cufftExecC2C + the host fftshift/Re/min-max
+ * (src/rx_host_corr.cu:74-85,205-206,219-235). Grids the integration's V by baseline
+ * (dy, dx) = ((qa-qb) mod ng, (pa-pb) mod ng) in exact integer arithmetic, runs a forward,
+ * unnormalised ng x ng FP32 FFT per channel in LDS, and writes
+ *   d_img [n_chan][ng][ng] f32 = Re F[(y+ng/2)%ng][(x+ng/2)%ng], d_vmax/d_vmin [n_chan]. */
+int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_img,
+                float* d_vmax, float* d_vmin);
+
+/* Plain per-channel-launch form (baseline): one corner-turn launch and one full-square GEMM
+ * launch per channel, exactly the reference's 4-real-product math (cgemm.cuh:114-119) on the
+ * whole square, no lower-triangle saving, no persistence. Output d_vre/d_vim [n_chan][N][N]
+ * int32 (overwrite). d_tiles is scratch of mixer_tiles_bytes(cfg, n_tb). */
+int mixer_xcorr_plain(void* stream, const mixer_array_cfg* cfg, const uint8_t* d_heaps, int n_tb,
+                      int8_t* d_tiles, int32_t* d_vre, int32_t* d_vim);
+
+/* ---- engine: the reference's gpu_thread + publish loop as a context -------------------------
+ * (src/rx_host_corr.cu:143-175 ingest/correlate, :213-239 monitor/dump). */
+typedef struct mixer_ctx mixer_ctx;
+
+typedef struct mixer_stats {
+    uint64_t heaps_in;      /* packets accepted */
+    uint64_t time_blocks;   /* complete time blocks correlated */
+    uint64_t vupdates;      /* V-updates launched (corner-turn + GEMM) */
+    uint64_t integrations;  /* integrations completed (V + image ready) */
+    uint64_t imissed;       /* time blocks dropped: no free slot (src/rx_host_corr.cu:140) */
+    uint64_t misalign;      /* submissions whose first seq % n_elem != 0 (src/rx_host_corr.cu:135) */
+    uint64_t seq_gaps;      /* time blocks dropped because a seq inside them was missing */
+    uint64_t wrap128;       /* corner-turn workgroups that saw an imaginary -128 */
+    double last_dump_latency_ms; /* arrival of the integration's last heap -> V/image ready */
+} mixer_stats;
+
+int mixer_create(const mixer_array_cfg* cfg, int device, mixer_ctx** out);
+int mixer_destroy(mixer_ctx* ctx);
+int mixer_get_cfg(mixer_ctx* ctx, mixer_array_cfg* cfg);
+
+/* Input seam (src/rx_host_corr.cu:136-141,159). pkts points to npkts packets at a pitch of
+ * `pitch` bytes in host memory (pinned memory gives an asynchronous copy). npkts must be a
+ * multiple of n_elem and the first seq a multiple of n_elem; otherwise the leading partial
+ * group is skipped and misalign is counted, as the reference's drain thread does. Packets
+ * are checked for consecutive seq; a time block with a gap is dropped (seq_gaps). The header
+ * strip is a hipMemcpy2DAsync on the context's copy stream. *token receives a completion
+ * token: the caller's buffer is free for reuse once mixer_token_done/wait reports it (the
+ * reference returns its packets to the NIC before the copy has run; this API does not).
+ * arrival_ns (steady clock) of the last packet feeds the dump-latency statistics. */
+int mixer_submit_heaps(mixer_ctx* ctx, const uint8_t* pkts, size_t pitch, int npkts,
+                       uint64_t arrival_ns, uint64_t* token);
+/* Device-resident input: header-stripped payloads [n_tb][n_elem][payload] already in HBM
+ * (the replayed-ring benchmark path). The buffer must stay valid until the token completes. */
+int mixer_submit_device(mixer_ctx* ctx, const uint8_t* d_payloads, int n_tb, uint64_t arrival_ns,
+                        uint64_t* token);
+int mixer_token_done(mixer_ctx* ctx, uint64_t token); /* 1 done, 0 pending, <0 error */
+int mixer_token_wait(mixer_ctx* ctx, uint64_t token);
+int mixer_sync(mixer_ctx* ctx);                       /* wait for all submitted work */
+
+/* Output seams. mixer_dump copies the most recently completed integration's V cube
+ * ([n_chan][2][n_elem][n_elem] f32, the reference's --dump record) to host memory; with
+ * cumulative=1 it copies the cube summed over all integrations since creation, which is the
+ * reference's record (requires MIXER_CFG_CUMULATIVE). *integ_index receives the number of
+ * completed integrations the snapshot covers (0 = none yet, nothing copied). The snapshot is
+ * consistent: it is taken between integrations, never while one is being written. */
+int mixer_dump(mixer_ctx* ctx, float* host_out, int cumulative, uint64_t* integ_index);
+/* The latest image cube: img [n_chan][ng][ng], vmax/vmin [n_chan] (any may be NULL). */
+int mixer_read_image(mixer_ctx* ctx, float* img, float* vmax, float* vmin, uint64_t* integ_index);
+
+/* Corr32 shared-memory image cube (src/rx_host_corr.cu:97-105): {u32 magic=0x43523343, w, h,
+ * nch; u64 write_seq; f32 img[nch*h*w]; f32 vmax[nch]; f32 vmin[nch]} = 525336 bytes at the
+ * default geometry. write_seq has bit 63 set while a frame is being written and holds
+ * frame+1 (release order) once it is complete, so readers that check the bit never see a torn
+ * frame and the reference's peek32 viewer reads the region unchanged. */
+size_t mixer_corr32_bytes(const mixer_array_cfg* cfg);
+int mixer_corr32_init(void* region, const mixer_array_cfg* cfg);
+int mixer_publish_image(mixer_ctx* ctx, void* region, size_t region_bytes);
+
+int mixer_get_stats(mixer_ctx* ctx, mixer_stats* st);
+
+/* Kernel timing on the context's compute stream: when enabled, HIP events bracket every
+ * launch of each pipeline stage. Stage ids: 0 corner-turn, 1 GEMM, 2 integrate, 3 image. */
+int mixer_set_timing(mixer_ctx* ctx, int enable);
+int mixer_get_timing(mixer_ctx* ctx, int stage, double* total_ms, uint64_t* launches);
+void* mixer_compute_stream(mixer_ctx* ctx);
+
+/* ---- HIP plumbing for hosts without a HIP binding of their own -------------------------------
+ * (no reference counterpart: the reference calls the CUDA runtime directly). The Python tests
+ * and bench.py reach device memory, streams and events only through these, so one HIP
+ * runtime serves the whole process. kind: 0 host->device, 1 device->host, 2 device->device. */
+int mixer_set_device(int device);
+int mixer_dev_alloc(size_t bytes, void** ptr);
+int mixer_dev_free(void* ptr);
+int mixer_host_alloc(size_t bytes, void** ptr); /* pinned host memory */
+int mixer_host_free(void* ptr);
+int mixer_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int mixer_memset(void* ptr, int value, size_t bytes, void* stream);
+int mixer_stream_create(void** stream);
+int mixer_stream_destroy(void* stream);
+int mixer_stream_sync(void* stream);
+int mixer_event_create(void** ev);
+int mixer_event_destroy(void* ev);
+int mixer_event_record(void* ev, void* stream);
+int mixer_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises on stop */
+int mixer_device_sync(void);
+
+/* ---- synthetic F-engine source (replaces the DAQIRI capture and tx_host) ------------------
+ * Signal model of k_sky (src/tx_host.cu:41-60) plus seeded noise: element a at
+ * (px, qy) = (a % grid_x, a / grid_x); x_a(c,t) = clip127(rint(25*[2 + Apl(c) e^{i pi (px l + qy m)}
+ * + sigma (n_r + i n_i)])), Apl(c) = 0.71*2*sqrt(2c/(nch_gen-1)), n ~ N(0,1) from a counter-based
+ * generator keyed by (seed, element, channel, absolute sample). n_src extra point sources
+ * (l_k, m_k, amp_k) may be added. Packets are written for seq in [seq0, seq0+npkts) at
+ * `pitch` bytes apart: header (seq big-endian at seq_byte) + planar payload. */
+typedef struct mixer_sky {
+    double planet_l, planet_m; /* planet direction cosines */
+    double sigma;              /* noise std in units of the 25-LSB scale (0 = noise-free) */
+    double star_amp;           /* 2.0 in the reference */
+    double planet_scale;       /* 0.71 in the reference (planet amplitude factor) */
+    int32_t nch_gen;           /* 256: the generator's channel sweep (src/config.h:28) */
+    int32_t n_src;             /* extra point sources, 0..4 */
+    double src_l[4], src_m[4], src_amp[4];
+} mixer_sky;
+
+void mixer_default_sky(mixer_sky* sky);
+int mixer_synth_packets(const mixer_array_cfg* cfg, const mixer_sky* sky, uint64_t seed,
+                        uint64_t seq0, int npkts, uint8_t* out, size_t pitch, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIXER_H */

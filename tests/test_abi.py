@@ -1,0 +1,90 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol include/mixer.h
+declares, and its host-only entry points (config, wire sizes, synthetic source, Corr32 size)
+behave; the synthetic source is checked against the oracle's k_sky restatement."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mixer_amd as M
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "mixer.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(mixer_[a-z0-9_]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = M.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(L, n), n
+    bound = {s[0] for s in M.SIGNATURES}
+    assert set(names) == bound, set(names) ^ bound
+
+
+def test_default_cfg_is_reference_constants():
+    c = M.default_cfg()
+    # src/config.h:26-38 and src/rx_host_corr.cu:47,51
+    assert (c.n_elem, c.grid_x, c.n_chan, c.t_pkt, c.hdr_bytes, c.seq_byte, c.ng, c.pkt_pitch) == (
+        256, 16, 128, 32, 64, 48, 32, 16384)
+    assert M.lib().mixer_payload_bytes(C.byref(c)) == 8192
+    assert M.lib().mixer_wire_bytes(C.byref(c)) == 8256
+    assert M.lib().mixer_validate_cfg(C.byref(c)) == M.OK
+    assert M.corr32_bytes(c) == 525336  # sizeof(Corr32) at the reference geometry
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("n_elem", 48, M.ERR_ARG), ("n_elem", 1056, M.ERR_ARG), ("n_elem", 288, M.ERR_ARG),
+    ("t_pkt", 16, M.ERR_ARG), ("k_integ", 65536, M.ERR_OVERFLOW), ("k_update", 100, M.ERR_ARG),
+    ("ng", 16, M.ERR_ARG), ("ng", 48, M.ERR_ARG), ("grid_x", 7, M.ERR_ARG), ("pkt_pitch", 8000, M.ERR_ARG),
+])
+def test_validate_rejects(field, value, status):
+    c = M.default_cfg()
+    setattr(c, field, value)
+    if field == "k_integ":
+        c.k_update = 1024
+    assert M.lib().mixer_validate_cfg(C.byref(c)) == status
+
+
+def test_validate_accepts_configs():
+    for n, gx, ch, k, ng in [(256, 16, 128, 8192, 32), (1024, 32, 256, 4096, 64), (64, 8, 16, 1024, 32),
+                             (512, 16, 200, 16384, 64), (128, 16, 64, 2048, 32)]:
+        c = M.default_cfg(n_elem=n, grid_x=gx, n_chan=ch, k_integ=k, k_update=k, ng=ng,
+                          pkt_pitch=max(16384, 64 + 64 * ch))
+        assert M.lib().mixer_validate_cfg(C.byref(c)) == M.OK, (n, ch)
+
+
+def test_synth_noise_free_matches_k_sky_restatement():
+    """sigma = 0: every payload equals the oracle's k_sky restatement (src/tx_host.cu:41-60)."""
+    c = M.default_cfg(n_elem=64, grid_x=8, n_chan=16)
+    sky = M.default_sky(sigma=0.0, planet_l=0.3, planet_m=-0.45)
+    pk = M.synth_packets(c, sky, seed=1, seq0=64 * 5, npkts=128)
+    for i in range(128):
+        seq = 64 * 5 + i
+        assert O.read_seq(np.ascontiguousarray(pk[i])) == seq
+        assert (pk[i, :48] == 0).all() and (pk[i, 52:64] == 0).all()
+        exp = O.sky_payload(seq % 64, 8, 16, 32, 0.3, -0.45)
+        assert np.array_equal(pk[i, 64:], exp), i
+
+
+def test_synth_noise_is_seeded_and_gaussian():
+    c = M.default_cfg(n_elem=32, grid_x=8, n_chan=64)
+    sky = M.default_sky(sigma=0.5, planet_scale=0.0, star_amp=0.0)
+    a = M.synth_packets(c, sky, seed=7, seq0=0, npkts=256)
+    b = M.synth_packets(c, sky, seed=7, seq0=0, npkts=256, nthreads=3)
+    d = M.synth_packets(c, sky, seed=8, seq0=0, npkts=256)
+    assert np.array_equal(a, b) and not np.array_equal(a, d)
+    v = a[:, 64:].view(np.int8).astype(np.float64)
+    assert abs(v.mean()) < 0.2 and abs(v.std() - 12.5) < 0.3  # 25 * sigma LSB
+    # noise is keyed by absolute (element, channel, sample): a later start reproduces the same packets
+    e = M.synth_packets(c, sky, seed=7, seq0=32, npkts=224)
+    assert np.array_equal(e, a[32:])

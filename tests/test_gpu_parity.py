@@ -1,0 +1,300 @@
+"""GPU parity of the HIP path against the oracle, through the C ABI.
+
+Bit-exact for the corner-turn bytes and the int32 V; the f32 integration within one int32->f32
+rounding (<= 2^-24 relative, tolerance 1e-6 relative to FP64); images against the FP64 grid +
+FP64 DFT oracle within 1e-5 of the channel's peak |image|.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mixer_amd as M
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(0x4D495839)
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_heaps(n_tb, N, C, lo=-127, hi=127, rng=RNG):
+    return rng.integers(lo, hi + 1, size=(n_tb, N, 64 * C), dtype=np.int16).astype(np.int8).view(np.uint8)
+
+
+def cfg_for(N, C, k_update, gx=None, ng=None, **kw):
+    gx = gx or (16 if N % 16 == 0 and N // 16 <= 16 else 32)
+    gy = N // gx
+    ng = ng or max(32, 1 << int(np.ceil(np.log2(2 * max(gx, gy)))))
+    return M.default_cfg(n_elem=N, grid_x=gx, n_chan=C, k_update=k_update, k_integ=k_update, ng=ng,
+                         pkt_pitch=max(16384, 64 + 64 * C), **kw)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert M.device_count() >= 1, "no gfx950 device visible to libmixer"
+    M.check(M.lib().mixer_set_device(0))
+    yield
+
+
+# ---------------------------------------------------------------- corner-turn
+@pytest.mark.parametrize("N,C,n_tb", [(256, 128, 4), (64, 40, 3), (32, 1, 2), (1024, 8, 2)])
+def test_corner_turn_bytes(N, C, n_tb):
+    log(f"corner_turn N={N} C={C} n_tb={n_tb}")
+    cfg = cfg_for(N, C, 32 * n_tb)
+    bb = rand_heaps(n_tb, N, C, -128, 127)
+    d_bb = M.DeviceBuffer.from_array(bb)
+    d_t = M.DeviceBuffer(M.tiles_bytes(cfg, n_tb))
+    d_w = M.DeviceBuffer(4)
+    d_w.zero()
+    M.corner_turn(cfg, d_bb, n_tb, d_t, d_w)
+    t = d_t.download(np.int8, (C, n_tb, 2, N, 32))
+    are, aim, _, _ = O.unpack(bb, n_tb, N, C, 32)
+    # oracle [C][N][K] -> tiles [C][tb][N][32]
+    exp_re = are.reshape(C, N, n_tb, 32).transpose(0, 2, 1, 3)
+    exp_im = aim.reshape(C, N, n_tb, 32).transpose(0, 2, 1, 3)
+    assert np.array_equal(t[:, :, 0], exp_re) and np.array_equal(t[:, :, 1], exp_im)
+    w = d_w.download(np.uint32, (1,))[0]
+    assert w > 0  # random bytes include imaginary -128
+    # clean input: no wrap flagged
+    bb2 = rand_heaps(n_tb, N, C)
+    d_bb.upload(bb2)
+    d_w.zero()
+    M.corner_turn(cfg, d_bb, n_tb, d_t, d_w)
+    assert d_w.download(np.uint32, (1,))[0] == 0
+
+
+# ---------------------------------------------------------------- GEMM
+def gpu_v(cfg, bb, n_tb, accumulate_parts=1):
+    """corner-turn + xcorr over n_tb blocks, split into accumulate_parts V-updates."""
+    N, C = cfg.n_elem, cfg.n_chan
+    per = n_tb // accumulate_parts
+    c2 = cfg_for(N, C, 32 * per, gx=cfg.grid_x, ng=cfg.ng)
+    d_acc = M.DeviceBuffer(M.acc_bytes(c2))
+    d_t = M.DeviceBuffer(M.tiles_bytes(c2, per))
+    for p in range(accumulate_parts):
+        d_bb = M.DeviceBuffer.from_array(np.ascontiguousarray(bb[p * per:(p + 1) * per]))
+        M.corner_turn(c2, d_bb, per, d_t)
+        M.xcorr(c2, d_t, per, d_acc, accumulate=p > 0)
+    acc = d_acc.download(np.int32, (M.acc_bytes(c2) // 4,))
+    return c2, d_acc, acc
+
+
+@pytest.mark.parametrize("N,C,n_tb,parts", [
+    (256, 4, 8, 1), (256, 3, 16, 2), (64, 5, 4, 1), (128, 3, 6, 3), (160, 2, 4, 1), (32, 2, 3, 1),
+    (96, 2, 2, 1), (384, 2, 2, 1), (512, 2, 3, 1), (1024, 1, 2, 1),
+])
+def test_xcorr_bit_exact(N, C, n_tb, parts):
+    log(f"xcorr N={N} C={C} n_tb={n_tb} parts={parts}")
+    cfg = cfg_for(N, C, 32 * n_tb)
+    bb = rand_heaps(n_tb, N, C)
+    c2, _, acc = gpu_v(cfg, bb, n_tb, parts)
+    vre, vim = M.unpack_acc(acc, c2)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
+
+
+def test_xcorr_extremes_and_single_element():
+    """All +-127 (int32 headroom) and one non-zero element."""
+    N, C, n_tb = 256, 2, 64
+    bb = np.where(RNG.random((n_tb, N, 64 * C)) < 0.5, 127, -127).astype(np.int8).view(np.uint8)
+    cfg = cfg_for(N, C, 32 * n_tb)
+    c2, _, acc = gpu_v(cfg, bb, n_tb)
+    vre, vim = M.unpack_acc(acc, c2)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
+    assert vre[0, 0, 0] == 2 * 127 * 127 * 32 * n_tb
+    bb = np.zeros((2, N, 64 * C), np.uint8)
+    bb[:, 77, :] = rand_heaps(2, 1, C)[:, 0, :]
+    c2, _, acc = gpu_v(cfg_for(N, C, 64), bb, 2)
+    vre, vim = M.unpack_acc(acc, c2)
+    ore, oim = O.v_full_from_heaps(bb, 2, N, C, 32)
+    assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
+    assert np.count_nonzero(vre) == C and np.count_nonzero(vim) == 0
+
+
+def test_xcorr_minus128_lower_triangle_matches_reference():
+    """With imaginary -128 present the reference's V is not Hermitian; the lower triangle and
+    the diagonal tiles stay bit-identical to it."""
+    N, C, n_tb = 128, 2, 4
+    bb = rand_heaps(n_tb, N, C, -128, 127)
+    c2, _, acc = gpu_v(cfg_for(N, C, 32 * n_tb), bb, n_tb)
+    nb = N // 32
+    a = acc.reshape(-1, C, nb * (nb + 1) // 2, 2, 32, 32).astype(np.int64).sum(0)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    t = 0
+    for I in range(nb):
+        for J in range(I + 1):
+            assert np.array_equal(a[:, t, 0], ore[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32])
+            assert np.array_equal(a[:, t, 1], oim[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32])
+            t += 1
+
+
+def test_plain_form_full_square():
+    N, C, n_tb = 256, 3, 4
+    cfg = cfg_for(N, C, 32 * n_tb)
+    bb = rand_heaps(n_tb, N, C, -128, 127)  # the full square reproduces the reference even with -128
+    d_bb = M.DeviceBuffer.from_array(bb)
+    d_t = M.DeviceBuffer(M.tiles_bytes(cfg, n_tb))
+    d_r = M.DeviceBuffer(C * N * N * 4)
+    d_i = M.DeviceBuffer(C * N * N * 4)
+    M.xcorr_plain(cfg, d_bb, n_tb, d_t, d_r, d_i)
+    vre = d_r.download(np.int32, (C, N, N))
+    vim = d_i.download(np.int32, (C, N, N))
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
+
+
+# ---------------------------------------------------------------- integrate
+def test_integrate_f32_and_cumulative():
+    N, C, n_tb = 256, 3, 8
+    bb = rand_heaps(n_tb, N, C)
+    c2, d_acc, acc = gpu_v(cfg_for(N, C, 32 * n_tb), bb, n_tb)
+    d_v = M.DeviceBuffer(C * 2 * N * N * 4)
+    M.integrate(c2, d_acc, d_v)
+    v = d_v.download(np.float32, (C, 2, N, N))
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    exact = np.stack([ore, oim], axis=1)
+    assert np.array_equal(v, exact.astype(np.float32))  # one rounding of the exact int32 sum
+    rel = np.abs(v.astype(np.float64) - exact) / np.maximum(np.abs(exact), 1)
+    assert rel.max() <= 1e-6
+    # cumulative: the reference's Vcube += (float)Vi (rx_host_corr.cu:93-94) over two integrations
+    M.integrate(c2, d_acc, d_v, accumulate=True)
+    v2 = d_v.download(np.float32, (C, 2, N, N))
+    assert np.array_equal(v2, exact.astype(np.float32) + exact.astype(np.float32))
+
+
+# ---------------------------------------------------------------- image
+@pytest.mark.parametrize("N,gx,C,ng,n_tb", [(256, 16, 4, 32, 4), (1024, 32, 2, 64, 2), (64, 8, 3, 32, 2)])
+def test_image_vs_fp64(N, gx, C, ng, n_tb):
+    log(f"image N={N} gx={gx} C={C} ng={ng}")
+    cfg = cfg_for(N, C, 32 * n_tb, gx=gx, ng=ng)
+    sky = M.default_sky(sigma=0.5, planet_l=0.5, planet_m=0.25)
+    pk = M.synth_packets(cfg, sky, seed=3, seq0=0, npkts=n_tb * N)
+    bb = np.ascontiguousarray(pk[:, 64:]).reshape(n_tb, N, 64 * C)
+    c2, d_acc, acc = gpu_v(cfg, bb, n_tb)
+    d_img = M.DeviceBuffer(C * ng * ng * 4)
+    d_mx = M.DeviceBuffer(C * 4)
+    d_mn = M.DeviceBuffer(C * 4)
+    M.image(c2, d_acc, d_img, d_mx, d_mn)
+    img = d_img.download(np.float32, (C, ng, ng))
+    vmax = d_mx.download(np.float32, (C,))
+    vmin = d_mn.download(np.float32, (C,))
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    oimg, omax, omin, _, _ = O.image_from_v(ore, oim, gx, ng)
+    scale = np.abs(oimg).reshape(C, -1).max(axis=1)
+    err = (np.abs(img.astype(np.float64) - oimg).reshape(C, -1).max(axis=1) / scale)
+    log(f"  image max error / peak = {err.max():.3e}")
+    assert err.max() <= 1e-5
+    assert np.array_equal(vmax, img.reshape(C, -1).max(1)) and np.array_equal(vmin, img.reshape(C, -1).min(1))
+    # star at the centre
+    for c in range(C):
+        assert np.unravel_index(np.argmax(img[c]), (ng, ng)) == (ng // 2, ng // 2)
+
+
+# ---------------------------------------------------------------- engine
+def heaps_of(pk, N, C):
+    return np.ascontiguousarray(pk[:, 64:64 + 64 * C]).reshape(-1, N, 64 * C)
+
+
+def test_engine_host_packets_dump_and_image():
+    N, C, k_upd, upd = 256, 8, 256, 4
+    cfg = cfg_for(N, C, k_upd, flags=M.CFG_CUMULATIVE)
+    cfg.k_integ = k_upd * upd
+    n_tb = cfg.k_integ // 32
+    sky = M.default_sky(sigma=0.5)
+    pitch = 1024  # > wire bytes (576): exercises the header strip at a pitch
+    pk = M.synth_packets(cfg, sky, seed=11, seq0=0, npkts=2 * n_tb * N, pitch=pitch)
+    with M.Engine(cfg) as e:
+        # feed in uneven pieces: a group boundary inside a call and one spanning calls
+        cuts = [0, 3 * N, 3 * N + 5 * N, 20 * N, 2 * n_tb * N]
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            tok = e.submit_heaps(np.ascontiguousarray(pk[a:b]), pitch, b - a)
+            e.token_wait(tok)
+        e.sync()
+        v, idx = e.dump()
+        vc, _ = e.dump(cumulative=True)
+        img, vmax, vmin, _ = e.read_image()
+        st = e.stats()
+    assert idx == 2 and st.integrations == 2 and st.vupdates == 2 * upd and st.misalign == 0
+    bb = heaps_of(pk, N, C)
+    r1, i1 = O.v_full_from_heaps(np.ascontiguousarray(bb[:n_tb]), n_tb, N, C, 32)
+    r2, i2 = O.v_full_from_heaps(np.ascontiguousarray(bb[n_tb:]), n_tb, N, C, 32)
+    assert np.array_equal(v, np.stack([r2, i2], 1).astype(np.float32))
+    assert np.array_equal(vc, np.stack([r1, i1], 1).astype(np.float32) + np.stack([r2, i2], 1).astype(np.float32))
+    oimg, _, _, _, _ = O.image_from_v(r2, i2, 16, 32)
+    assert np.abs(img - oimg).max() / np.abs(oimg).max() <= 1e-5
+
+
+def test_engine_misalign_and_gap():
+    N, C = 64, 4
+    cfg = cfg_for(N, C, 64)
+    pitch = 640
+    sky = M.default_sky()
+    pk = M.synth_packets(cfg, sky, seed=5, seq0=N - 7, npkts=7 + 4 * N, pitch=pitch)
+    # drop one packet inside the third block
+    keep = np.ones(len(pk), bool)
+    keep[7 + 2 * N + 10] = False
+    pk2 = np.ascontiguousarray(pk[keep])
+    with M.Engine(cfg) as e:
+        e.token_wait(e.submit_heaps(pk2, pitch, len(pk2)))
+        e.sync()
+        st = e.stats()
+        v, idx = e.dump()
+    assert st.misalign == 1 and st.seq_gaps == 1 and st.time_blocks == 2 and idx == 1
+    # blocks 0,1 and 3 were complete; the first integration is blocks 0,1
+    bb = heaps_of(pk[7:], N, C)
+    r, i = O.v_full_from_heaps(np.ascontiguousarray(bb[:2]), 2, N, C, 32)
+    assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
+
+
+def test_engine_device_submit_matches_host_submit():
+    N, C = 256, 16
+    cfg = cfg_for(N, C, 512)
+    cfg.k_integ = 1024
+    n_tb = 32
+    pk = M.synth_packets(cfg, M.default_sky(), seed=9, seq0=0, npkts=n_tb * N)
+    bb = heaps_of(pk, N, C)
+    d = M.DeviceBuffer.from_array(bb)
+    with M.Engine(cfg) as e:
+        tb_bytes = N * 64 * C
+        # uneven device pieces: 5 + 20 + 7 blocks
+        e.submit_device(d, 5, offset=0)
+        e.submit_device(d, 20, offset=5 * tb_bytes)
+        e.token_wait(e.submit_device(d, 7, offset=25 * tb_bytes))
+        e.sync()
+        v, idx = e.dump()
+    r, i = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    assert idx == 1 and np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
+
+
+def test_full_size_cfg2_channels_subset():
+    """The 256-element x 128-channel x 8192-sample integration, device-resident: channels
+    0, 77 and 127 bit-exact against the CPU correlator, and Hermitian/diagonal invariants on all."""
+    N, C, K = 256, 128, 8192
+    cfg = cfg_for(N, C, K)
+    n_tb = K // 32
+    pk = M.synth_packets(cfg, M.default_sky(), seed=0x4D495832, seq0=0, npkts=n_tb * N, nthreads=16)
+    bb = heaps_of(pk, N, C)
+    del pk
+    d = M.DeviceBuffer.from_array(bb)
+    with M.Engine(cfg) as e:
+        e.token_wait(e.submit_device(d, n_tb))
+        e.sync()
+        v, idx = e.dump()
+        img, _, _, _ = e.read_image()
+    assert idx == 1
+    for c0 in (0, 77, 127):
+        r, i, of = O.xengine(bb, n_tb, N, C, 32, c0, c0 + 1, nthreads=16)
+        assert of == 0
+        assert np.array_equal(v[c0, 0], r[0].astype(np.float32)) and np.array_equal(v[c0, 1], i[0].astype(np.float32))
+    assert np.array_equal(v[:, 0], v[:, 0].transpose(0, 2, 1))
+    assert np.array_equal(v[:, 1], -v[:, 1].transpose(0, 2, 1))
+    assert (np.diagonal(v[:, 1], axis1=1, axis2=2) == 0).all()
+    assert (np.diagonal(v[:, 0], axis1=1, axis2=2) > 0).all()
+    # star at the centre of every channel's image
+    assert all(np.unravel_index(np.argmax(img[c]), (32, 32)) == (16, 16) for c in range(C))
