@@ -214,6 +214,13 @@ def main():
     log(f"[bench] latency p50 {latency['p50_ms']:.3f} ms p99 {latency['p99_ms']:.3f} ms over {len(lat)} integrations")
     eng.close()
 
+    # ---- calibration copy: one device-to-device copy of one ring (a known-size HBM stream that
+    # rocprofv3 counter runs use to scale FETCH_SIZE / WRITE_SIZE) ----
+    cal = M.DeviceBuffer(alg["payload_bytes"])
+    M.check(L.mixer_memcpy(cal.ptr, rings[1].ptr, alg["payload_bytes"], M.D2D, None))
+    M.check(L.mixer_device_sync())
+    cal.free()
+
     # ---- plain per-channel-launch GPU form ----
     plain = None
     if not args.no_plain:
