@@ -1,0 +1,346 @@
+// batch.cpp -- batched correlation of many independent arrays (mixer_batch_*): one corner-turn
+// launch, one persistent correlator launch over a cost-ordered work list, one integrate +
+// gridding launch and one image launch per run, captured once as a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "mixer.h"
+
+using namespace mixer;
+
+struct mixer_batch {
+    int device = 0;
+    std::vector<mixer_array_cfg> cfg;
+    mixer_batch_tuning tun{};
+    std::vector<BatchJobDev> jobs;
+    std::vector<int> slot_cap;  // accumulator capacity (time slots) per job
+    BatchJobDev* d_jobs = nullptr;
+    int4* d_items = nullptr;
+    int items_cap = 0, n_items = 0, n_wgs = 0;
+    int* d_counter = nullptr;
+    uint32_t* d_wrap = nullptr;
+    int ct_total = 0, int_total = 0, img_total = 0, max_ng = 8;
+    std::vector<void*> allocs;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool have_inputs = false;
+    uint64_t payload = 0, exec_ops = 0, dev_bytes = 0;
+};
+
+namespace {
+
+int balloc(mixer_batch* b, void** p, size_t bytes) {
+    MIXER_HIP(hipMalloc(p, bytes));
+    b->allocs.push_back(*p);
+    b->dev_bytes += bytes;
+    return MIXER_OK;
+}
+
+int items_per_slot(const mixer_array_cfg& c, int pack) {
+    const int nb = c.n_elem / 32;
+    if (nb > 8) return (int)sblocks_for(nb).size() * c.n_chan;
+    const int g = pack == 1 ? 1 : batch_fold_groups(nb);
+    return (c.n_chan + g - 1) / g;
+}
+
+int slots_for(const mixer_array_cfg& c, int pack, int tb_target) {
+    const int n_tb = c.k_integ / c.t_pkt;
+    if (items_per_slot(c, pack) >= 512) return 1;
+    int s = (n_tb + tb_target - 1) / tb_target;
+    return std::max(1, std::min({s, 8, n_tb}));
+}
+
+void destroy_graph(mixer_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int enqueue(mixer_batch* b) {
+    MIXER_HIP(launch_batch_reset(b->s, b->d_counter));
+    MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
+    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs));
+    MIXER_HIP(launch_batch_integrate(b->s, b->d_jobs, (int)b->jobs.size(), b->int_total, b->max_ng));
+    MIXER_HIP(launch_batch_image(b->s, b->d_jobs, (int)b->jobs.size(), b->img_total, b->max_ng));
+    return MIXER_OK;
+}
+
+// (Re)build the work list, the time split of each job and the graph for the current tuning.
+int plan(mixer_batch* b) {
+    MIXER_HIP(hipStreamSynchronize(b->s));
+    destroy_graph(b);
+    const int pack = b->tun.pack, tbt = b->tun.tb_target > 0 ? b->tun.tb_target : 128;
+    struct Item { int4 w; double cost; };
+    std::vector<Item> items;
+    b->exec_ops = 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const mixer_array_cfg& c = b->cfg[j];
+        BatchJobDev& J = b->jobs[j];
+        const int nb = J.nb, n_tb = J.n_tb;
+        J.n_slots = std::min(slots_for(c, pack, tbt), b->slot_cap[j]);
+        J.ch_per_item = pack == 1 ? 1 : batch_fold_groups(nb);
+        const int NT = nb * (nb + 1) / 2;
+        b->exec_ops += (uint64_t)NT * 1024 * 4 * 2 * (uint64_t)n_tb * 32 * c.n_chan;
+        for (int sl = 0; sl < J.n_slots; ++sl) {
+            const double kb = (double)n_tb / J.n_slots;
+            if (nb <= 8) {
+                const int g = pack == 1 ? 1 : batch_fold_groups(nb);
+                for (int c0 = 0; c0 < c.n_chan; c0 += g)
+                    items.push_back({make_int4((int)j, c0, 0, sl), kb * NT * std::min(g, c.n_chan - c0)});
+            } else {
+                const auto sb = sblocks_for(nb);
+                for (int ch = 0; ch < c.n_chan; ++ch)
+                    for (size_t q = 0; q < sb.size(); ++q) {
+                        const int tiles = sb[q].second + 7 <= sb[q].first ? 32 : 20;
+                        items.push_back({make_int4((int)j, ch, (int)q, sl), kb * tiles});
+                    }
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+    b->n_items = (int)items.size();
+    if (b->n_items > b->items_cap) {
+        if (b->d_items) {
+            (void)hipFree(b->d_items);
+            b->allocs.erase(std::find(b->allocs.begin(), b->allocs.end(), (void*)b->d_items));
+        }
+        b->items_cap = b->n_items;
+        MIXER_TRY(balloc(b, (void**)&b->d_items, sizeof(int4) * b->items_cap));
+    }
+    std::vector<int4> h(b->n_items);
+    for (int i = 0; i < b->n_items; ++i) h[i] = items[i].w;
+    MIXER_HIP(hipMemcpy(b->d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    MIXER_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJobDev) * b->jobs.size(),
+                        hipMemcpyHostToDevice));
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) cus = prop.multiProcessorCount;
+    b->n_wgs = std::max(1, std::min(b->n_items, cus));
+    if (b->tun.use_graph) {
+        MIXER_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->s, &g);
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        MIXER_HIP(e);
+        b->graph = g;
+        MIXER_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    return MIXER_OK;
+}
+
+void destroy_batch(mixer_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    destroy_graph(b);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixer_batch_tuning* t,
+                 mixer_batch* b) {
+    b->device = device;
+    MIXER_HIP(hipSetDevice(device));
+    if (t) b->tun = *t;
+    else mixer_batch_default_tuning(&b->tun);
+    MIXER_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    MIXER_TRY(balloc(b, (void**)&b->d_counter, sizeof(int)));
+    MIXER_TRY(balloc(b, (void**)&b->d_wrap, sizeof(uint32_t)));
+    MIXER_HIP(hipMemset(b->d_wrap, 0, sizeof(uint32_t)));
+    MIXER_TRY(balloc(b, (void**)&b->d_jobs, sizeof(BatchJobDev) * n_jobs));
+    int ct = 0, in = 0, im = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        mixer_array_cfg c = cfgs[j];
+        c.k_update = c.k_integ;
+        MIXER_TRY(mixer_validate_cfg(&c));
+        b->cfg.push_back(c);
+        BatchJobDev J;
+        memset(&J, 0, sizeof(J));
+        const int nb = c.n_elem / 32, NT = nb * (nb + 1) / 2, n_tb = c.k_integ / c.t_pkt;
+        J.n_elem = c.n_elem; J.n_chan = c.n_chan; J.n_tb = n_tb; J.grid_x = c.grid_x;
+        J.ng = c.ng; J.nb = nb; J.wrap = b->d_wrap;
+        // accumulator capacity: the most time slots any tuning candidate (tb_target >= 64) uses
+        const int cap = std::min(slots_for(c, 0, 64), 8);
+        b->slot_cap.push_back(cap);
+        MIXER_TRY(balloc(b, (void**)&J.tiles, mixer_tiles_bytes(&c, n_tb)));
+        MIXER_TRY(balloc(b, (void**)&J.acc, (size_t)cap * c.n_chan * NT * 2048 * sizeof(int32_t)));
+        MIXER_TRY(balloc(b, (void**)&J.v, (size_t)c.n_chan * 2 * c.n_elem * c.n_elem * sizeof(float)));
+        const size_t gbytes = (size_t)c.n_chan * c.ng * c.ng * 16;
+        MIXER_TRY(balloc(b, (void**)&J.G, gbytes));
+        MIXER_HIP(hipMemset(J.G, 0, gbytes));
+        MIXER_TRY(balloc(b, (void**)&J.img, ((size_t)c.n_chan * c.ng * c.ng + 2 * c.n_chan) * sizeof(float)));
+        MIXER_TRY(get_twiddles(c.ng, &J.tw));
+        if (nb > 8) {
+            XcorrPlan p;
+            MIXER_TRY(get_plan(c, &p));
+            J.sblocks = (const int2*)p.d_tasks;
+            J.n_sb = p.n_tasks;
+        }
+        J.ct_wg0 = ct; ct += n_tb * nb * ((c.n_chan + 7) / 8);
+        J.int_wg0 = in; in += nb * c.n_chan;
+        J.img_wg0 = im; im += c.n_chan;
+        b->max_ng = std::max(b->max_ng, c.ng);
+        b->payload += (uint64_t)n_tb * c.n_elem * mixer_payload_bytes(&c);
+        b->jobs.push_back(J);
+    }
+    b->ct_total = ct; b->int_total = in; b->img_total = im;
+    MIXER_HIP(set_batch_lds_limits(b->max_ng));
+    MIXER_HIP(hipDeviceSynchronize());
+    return plan(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void mixer_batch_default_tuning(mixer_batch_tuning* t) {
+    if (!t) return;
+    t->pack = 0;
+    t->tb_target = 128;
+    t->use_graph = 1;
+    t->reserved = 0;
+}
+
+int mixer_batch_create(const mixer_array_cfg* cfgs, int n_jobs, int device,
+                       const mixer_batch_tuning* tuning, mixer_batch** out) {
+    if (!cfgs || n_jobs <= 0 || !out) return MIXER_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MIXER_ERR_NODEV;
+    mixer_batch* b = new mixer_batch();
+    const int st = create_batch(cfgs, n_jobs, device, tuning, b);
+    if (st) {
+        destroy_batch(b);
+        return st;
+    }
+    *out = b;
+    return MIXER_OK;
+}
+
+int mixer_batch_destroy(mixer_batch* b) {
+    destroy_batch(b);
+    return MIXER_OK;
+}
+
+int mixer_batch_run(mixer_batch* b, const uint8_t* const* d_heaps) {
+    if (!b || !d_heaps) return MIXER_ERR_ARG;
+    MIXER_HIP(hipSetDevice(b->device));
+    bool changed = false;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        if (!d_heaps[j]) return MIXER_ERR_ARG;
+        if (b->jobs[j].heaps != d_heaps[j]) { b->jobs[j].heaps = d_heaps[j]; changed = true; }
+    }
+    if (changed) {
+        // the table is read by the graph's kernels at run time; update it in stream order
+        MIXER_HIP(hipStreamSynchronize(b->s));
+        MIXER_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJobDev) * b->jobs.size(),
+                            hipMemcpyHostToDevice));
+    }
+    b->have_inputs = true;
+    if (b->exec) return hip_status(hipGraphLaunch(b->exec, b->s));
+    return enqueue(b);
+}
+
+int mixer_batch_sync(mixer_batch* b) {
+    if (!b) return MIXER_ERR_ARG;
+    MIXER_HIP(hipSetDevice(b->device));
+    return hip_status(hipStreamSynchronize(b->s));
+}
+
+void* mixer_batch_stream(mixer_batch* b) { return b ? (void*)b->s : nullptr; }
+
+int mixer_batch_get_info(mixer_batch* b, mixer_batch_info* info) {
+    if (!b || !info) return MIXER_ERR_ARG;
+    info->n_jobs = (int)b->jobs.size();
+    info->n_items = b->n_items;
+    info->n_wgs = b->n_wgs;
+    info->graph = b->exec ? 1 : 0;
+    info->payload_bytes = b->payload;
+    info->exec_ops = b->exec_ops;
+    info->device_bytes = b->dev_bytes;
+    return MIXER_OK;
+}
+
+int mixer_batch_dump(mixer_batch* b, int job, float* host_v) {
+    if (!b || !host_v || job < 0 || job >= (int)b->jobs.size()) return MIXER_ERR_ARG;
+    MIXER_HIP(hipSetDevice(b->device));
+    MIXER_HIP(hipStreamSynchronize(b->s));
+    const mixer_array_cfg& c = b->cfg[job];
+    return hip_status(hipMemcpy(host_v, b->jobs[job].v,
+                                (size_t)c.n_chan * 2 * c.n_elem * c.n_elem * sizeof(float),
+                                hipMemcpyDeviceToHost));
+}
+
+int mixer_batch_read_image(mixer_batch* b, int job, float* img, float* vmax, float* vmin) {
+    if (!b || job < 0 || job >= (int)b->jobs.size()) return MIXER_ERR_ARG;
+    MIXER_HIP(hipSetDevice(b->device));
+    MIXER_HIP(hipStreamSynchronize(b->s));
+    const mixer_array_cfg& c = b->cfg[job];
+    const size_t n = (size_t)c.n_chan * c.ng * c.ng;
+    const float* d = b->jobs[job].img;
+    if (img) MIXER_HIP(hipMemcpy(img, d, n * 4, hipMemcpyDeviceToHost));
+    if (vmax) MIXER_HIP(hipMemcpy(vmax, d + n, c.n_chan * 4, hipMemcpyDeviceToHost));
+    if (vmin) MIXER_HIP(hipMemcpy(vmin, d + n + c.n_chan, c.n_chan * 4, hipMemcpyDeviceToHost));
+    return MIXER_OK;
+}
+
+int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots) {
+    if (!b || job < 0 || job >= (int)b->jobs.size()) return MIXER_ERR_ARG;
+    if (d_acc) *d_acc = b->jobs[job].acc;
+    if (n_slots) *n_slots = b->jobs[job].n_slots;
+    return MIXER_OK;
+}
+
+int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best) {
+    if (!b || !best || reps <= 0) return MIXER_ERR_ARG;
+    if (!b->have_inputs) return MIXER_ERR_STATE;
+    MIXER_HIP(hipSetDevice(b->device));
+    hipEvent_t e0, e1;
+    MIXER_HIP(hipEventCreate(&e0));
+    MIXER_HIP(hipEventCreate(&e1));
+    const mixer_batch_tuning start = b->tun;
+    mixer_batch_tuning win = start;
+    float best_ms = 1e30f;
+    int st = MIXER_OK;
+    for (int pack : {0, 1}) {
+        for (int tbt : {64, 128, 256, 512}) {
+            b->tun.pack = pack;
+            b->tun.tb_target = tbt;
+            b->tun.use_graph = start.use_graph;
+            if ((st = plan(b))) break;
+            auto run = [&]() { return b->exec ? hip_status(hipGraphLaunch(b->exec, b->s)) : enqueue(b); };
+            if ((st = run())) break;
+            if ((st = hip_status(hipEventRecord(e0, b->s)))) break;
+            for (int r = 0; r < reps && !st; ++r) st = run();
+            if (st || (st = hip_status(hipEventRecord(e1, b->s)))) break;
+            if ((st = hip_status(hipEventSynchronize(e1)))) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) { best_ms = ms; win = b->tun; }
+        }
+        if (st) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tun = st ? start : win;
+    MIXER_TRY(plan(b));
+    if (st) return st;
+    *best = win;
+    return MIXER_OK;
+}
+
+int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed) {
+    if (!d_ptr || bytes % 8) return MIXER_ERR_ARG;
+    return hip_status(launch_fill_random((hipStream_t)stream, (uint8_t*)d_ptr, bytes, seed));
+}
+
+}  // extern "C"

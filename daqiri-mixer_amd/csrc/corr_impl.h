@@ -1,0 +1,387 @@
+// corr_impl.h -- the correlator: lower-triangle complex int8 GEMM V += X X^H on
+// v_mfma_i32_32x32x32_i8 with int32 accumulation into a tile-packed accumulator (replaces
+// cgemm_blk_t<ACC>, src/cgemm.cuh:84-125).
+//   k_xcorr_fold<NB,V>  arrays of NB <= 8 blocks of 32 elements (N <= 256)
+//   k_xcorr_blk<NB,V>   arrays of NB = 12..32 blocks (N = 384..1024, N % 128 == 0)
+// V selects the time-loop pipeline (the plan's "prefetch" field):
+//   V = 1   one 32-sample block per barrier, staged global -> registers -> LDS one block ahead;
+//   V = K>1 K blocks per barrier (FOLD: K = V, BLOCKED: K = min(V, 3)), staged by direct
+//           global -> LDS loads (global_load_lds_dwordx4) one stage ahead into two separately
+//           declared LDS buffers, so the barrier and LDS-latency bubble at each stage boundary
+//           is paid once per K blocks and no staging registers are needed.
+#pragma once
+#include "device_common.h"
+#include "kernels.h"
+
+namespace mixer {
+
+#define MIXER_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
+
+template <int V, int NBLK>
+struct StageOf {
+    static constexpr int KB = V == 1 ? 1 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
+};
+
+// The MFMAs of one 32-sample block for tiles (RA, [A0,A1)) u (RB, [B0,B1)) from the LDS slab L
+// (NE elements, swizzled as lds_unit<NE>).
+template <int NE, int RA, int A0, int A1, int RB, int B0, int B1>
+__device__ __forceinline__ void block_mfma(const uint8_t* L, int lrow, int kg, v16i* ar, v16i* ai,
+                                           v16i* br, v16i* bi) {
+    constexpr int JLO = (RB >= 0 && B0 < A0) ? B0 : A0;
+    constexpr int JHI = (RB >= 0 && B1 > A1) ? B1 : A1;
+    const Frag fa = load_frag<NE>(L, RA, lrow, kg);
+    Frag fb = fa;
+    if constexpr (RB >= 0) fb = load_frag<NE>(L, RB, lrow, kg);
+    static_for<JLO, JHI>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        constexpr bool inA = J >= A0 && J < A1;
+        constexpr bool inB = RB >= 0 && J >= B0 && J < B1;
+        if constexpr (inA || inB) {
+            Frag b;
+            if constexpr (J == RA) b = fa;
+            else if constexpr (J == RB) b = fb;
+            else b = load_frag<NE>(L, J, lrow, kg);
+            if constexpr (inA) tile_step(fa, b, ar[J - A0], ai[J - A0]);
+            if constexpr (inB) tile_step(fb, b, br[J - B0], bi[J - B0]);
+        }
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// One wave's share of the correlator over the whole time range (n_kb 32-sample blocks from
+// src, one global slab of `slab` bytes per block). The staged LDS slab holds NBLK element
+// blocks (NE = 32*NBLK elements) per time block; GB maps a staged block slot to its element
+// block in the global slab; NTH threads with local ids ltid share the staging. Every wave of
+// the group runs this loop and meets the others at exactly one barrier per stage (plus one).
+// ------------------------------------------------------------------------------------------
+template <int V, int NE, int NBLK, int NTH, int RA, int A0, int A1, int RB, int B0, int B1, typename GB>
+__device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t slab,
+                                          uint8_t* lds0, uint8_t* lds1, int n_kb, GB gblk,
+                                          v16i* ar, v16i* ai, v16i* br, v16i* bi, int ltid) {
+    const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
+    if constexpr (V == 1) {
+        auto ident = [](int b) { return b; };
+        using St = Stager<NE, NBLK, NTH>;
+        St st;
+        auto gload = [&](const int8_t* sl) {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
+                    const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                    st.r[i] = *(const v4i*)(sl + (size_t)p * (slab >> 1) + gblk(b) * 1024 + eg * 16);
+                }
+            }
+        };
+        gload(src);
+        st.store(lds0, ident, ltid);
+        __syncthreads();
+        for (int kb = 0; kb < n_kb; ++kb) {
+            const bool more = kb + 1 < n_kb;
+            if (more) gload(src + (size_t)(kb + 1) * slab);
+            block_mfma<NE, RA, A0, A1, RB, B0, B1>((kb & 1) ? lds1 : lds0, lrow, kg, ar, ai, br, bi);
+            if (more) st.store((kb & 1) ? lds0 : lds1, ident, ltid);
+            __syncthreads();
+        }
+    } else {
+        constexpr int KB = StageOf<V, NBLK>::KB;
+        constexpr int NW = NTH / 64, NWIN = NBLK * 2;  // 1 KiB windows per time block
+        const int w = ltid >> 6, el = lane >> 1;
+        // lane l fills unit l of a window: element el, k-half (l & 1) ^ swizzle(el)
+        const int goff = el * 32 + ((((lane & 1) ^ ((el >> 2) ^ (el >> 3))) & 1) << 4);
+        const int n_st = (n_kb + KB - 1) / KB;
+        auto dma = [&](uint8_t* L, int stg) {
+            const int kb0 = stg * KB;
+            const int kbs = n_kb - kb0 < KB ? n_kb - kb0 : KB;
+            for (int win = w; win < kbs * NWIN; win += NW) {
+                const int kbl = win / NWIN, r = win % NWIN, p = r / NBLK, b = r % NBLK;
+                const int8_t* g = src + (size_t)(kb0 + kbl) * slab + (size_t)p * (slab >> 1) +
+                                  gblk(b) * 1024 + goff;
+                uint8_t* l = L + kbl * (NE * 64) + p * (NE * 32) + b * 1024;
+                __builtin_amdgcn_global_load_lds((const void*)g, MIXER_LDS_PTR(l), 16, 0, 0);
+            }
+        };
+        auto compute = [&](const uint8_t* L, int stg) {
+            const int kbs = n_kb - stg * KB < KB ? n_kb - stg * KB : KB;
+            static_for<0, KB>([&](auto kc) {
+                constexpr int kbl = decltype(kc)::value;
+                if (kbl < kbs)
+                    block_mfma<NE, RA, A0, A1, RB, B0, B1>(L + kbl * (NE * 64), lrow, kg, ar, ai, br, bi);
+            });
+        };
+        dma(lds0, 0);
+        __syncthreads();  // waits for this wave's DMA (vmcnt) before the barrier
+        for (int stg = 0; stg < n_st; stg += 2) {
+            if (stg + 1 < n_st) dma(lds1, stg + 1);
+            compute(lds0, stg);
+            __syncthreads();
+            if (stg + 1 >= n_st) break;
+            if (stg + 2 < n_st) dma(lds0, stg + 2);
+            compute(lds1, stg + 1);
+            __syncthreads();
+        }
+    }
+}
+
+// FOLD wave program: tiles (RA, [A0,A1)) u (RB, [B0,B1)) of the lower triangle of NB blocks;
+// acc points at the channel's tile-packed accumulator (tile t = I(I+1)/2 + J, 2048 ints).
+template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
+__device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, int32_t* __restrict__ acc,
+                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                          int ltid) {
+    constexpr int NA = A1 - A0;
+    constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
+    const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
+    v16i ar[NA], ai[NA], br[NBT], bi[NBT];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) { ar[j] = (v16i){0}; ai[j] = (v16i){0}; }
+#pragma unroll
+    for (int j = 0; j < NBT; ++j) { br[j] = (v16i){0}; bi[j] = (v16i){0}; }
+    corr_wave<V, NB * 32, NB, NTH, RA, A0, A1, RB, B0, B1>(src, (size_t)NB * 32 * 64, lds0, lds1,
+                                                           n_kb, [](int b) { return b; }, ar, ai,
+                                                           br, bi, ltid);
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+        int32_t* t = acc + (size_t)(RA * (RA + 1) / 2 + A0 + j) * 2048;
+        store_tile(ar[j], t, lrow, kg, accumulate);
+        store_tile(ai[j], t + 1024, lrow, kg, accumulate);
+    }
+    if constexpr (RB >= 0) {
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+            int32_t* t = acc + (size_t)(RB * (RB + 1) / 2 + B0 + j) * 2048;
+            store_tile(br[j], t, lrow, kg, accumulate);
+            store_tile(bi[j], t + 1024, lrow, kg, accumulate);
+        }
+    }
+}
+
+// Wave layout of the FOLD correlator. Row pair p = (p, NB-1-p) holds T_p = NB+1 tiles (the
+// middle row of an odd NB holds (NB+1)/2). NB < SPLIT_FROM: one wave per pair (up to 6 tiles,
+// which needs one wave per SIMD); NB >= SPLIT_FROM (6 by default, 2 in the batched persistent
+// kernel that runs two waves per SIMD): two waves per pair, split at column S = ceil(T_p/2) of the far row: wave A = (R2, [0,S)), wave B =
+// (R1, [0,R1]) u (R2, [S,R2]); waves 0..P-1 are the A halves, P..2P-1 the B halves, so with
+// waves placed on SIMD (w mod 4) each SIMD of an 8-block array gets 5 + 4 = 9 tiles.
+template <int NB, int SPLIT_FROM = 6>
+struct FoldLayout {
+    static constexpr int P = (NB + 1) / 2;
+    static constexpr bool SPLIT = NB >= SPLIT_FROM;
+    static constexpr int WAVES = SPLIT ? 2 * P : P;
+    static constexpr int NTH = 64 * WAVES;
+};
+
+template <int V, int NB, int W, int SPLIT_FROM = 6>
+__device__ __forceinline__ void fold_run(const int8_t* src, int32_t* acc, uint8_t* lds0,
+                                         uint8_t* lds1, int n_kb, int accumulate, int ltid) {
+    using Lay = FoldLayout<NB, SPLIT_FROM>;
+    constexpr int NTH = Lay::NTH;
+    if constexpr (!Lay::SPLIT) {
+        constexpr int R1 = W, R2 = NB - 1 - W;
+        if constexpr (R1 == R2)
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        else
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+    } else {
+        constexpr int p = W % Lay::P;
+        constexpr int R1 = p, R2 = NB - 1 - p;
+        constexpr int T = (R1 == R2) ? R1 + 1 : R1 + R2 + 2;
+        constexpr int S = (T + 1) / 2;
+        if constexpr (W < Lay::P) {
+            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        } else if constexpr (R1 == R2) {
+            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        } else if constexpr (S > R2) {
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        } else {
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        }
+    }
+}
+
+// LDS bytes of one stage buffer of the FOLD correlator
+template <int NB, int V>
+struct FoldLds {
+    static constexpr int BYTES = StageOf<V, NB>::KB * NB * 32 * 64;
+};
+
+// One channel (or time slot of it) of the FOLD correlator on the FoldLayout<NB>::NTH threads
+// with local ids ltid in [0, NTH) (the group's wave w = ltid / 64 runs wave program w).
+template <int NB, int V, int SPLIT_FROM = 6>
+__device__ __forceinline__ void fold_item(const int8_t* src, int32_t* acc, uint8_t* lds0,
+                                          uint8_t* lds1, int n_kb, int accumulate, int ltid) {
+    const int w = ltid >> 6;
+    static_for<0, FoldLayout<NB, SPLIT_FROM>::WAVES>([&](auto Wc) {
+        constexpr int W = decltype(Wc)::value;
+        if (w == W) fold_run<V, NB, W, SPLIT_FROM>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+    });
+}
+
+// FOLD correlator (arrays of NB <= 8 blocks of 32 elements). grid: x = channel, y = time slot.
+// The whole channel slab (NB blocks) is staged per stage into one of two LDS buffers.
+template <int NB, int V>
+__global__ __launch_bounds__(FoldLayout<NB>::NTH)
+void k_xcorr_fold(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc, int C, int n_tb,
+                  int n_slots, int accumulate) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    __shared__ __attribute__((aligned(16))) uint8_t lds0[FoldLds<NB, V>::BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds1[FoldLds<NB, V>::BYTES];
+    const int c = blockIdx.x, slot = blockIdx.y;
+    const int kb0 = (int)((long)n_tb * slot / n_slots);
+    const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
+    if (kb1 <= kb0) return;  // uniform over the workgroup
+    int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
+    const int8_t* src = tiles + ((size_t)c * n_tb + kb0) * (size_t)NB * 32 * 64;
+    fold_item<NB, V>(src, a, lds0, lds1, kb1 - kb0, accumulate, threadIdx.x);
+}
+
+// ------------------------------------------------------------------------------------------
+// BLOCKED correlator (arrays of NB = 12..32 blocks, NB % 4 == 0). A workgroup owns a
+// super-block of 4 tile rows x 8 tile columns of the lower triangle (rows I0..I0+3, I0 % 4 == 0;
+// columns J0..J0+7, J0 % 8 == 0); wave w (8 waves) owns the 2x2 tile block rows
+// I0 + 2(w/4) + {0,1}, columns J0 + 2(w%4) + {0,1}. With even row and column starts a wave
+// block is FULL (4 tiles), DIAG (3 tiles) or EMPTY, fixed per wave, so every path has a
+// compile-time tile set. The 12 row/column element blocks are staged per stage into LDS
+// (slots 0..3 rows, 4..11 columns). Super-blocks of one channel are mapped to one XCD
+// (dispatch is round-robin over the 8 XCDs) so the channel's slab is read from HBM about once
+// per XCD L2.
+// ------------------------------------------------------------------------------------------
+template <int V, int PAT, int RS, int CS, typename GB>
+__device__ __forceinline__ void blk_prog(const int8_t* __restrict__ src, size_t slab,
+                                         int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
+                                         int n_kb, GB gblk, int Ib, int Jb, int accumulate,
+                                         int ltid) {
+    // PAT 0 FULL: rows (RS, RS+1) x columns (CS, CS+1); 1 DIAG: (0,0), (1,0), (1,1); 2 EMPTY
+    const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
+    v16i ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) { ar[j] = (v16i){0}; ai[j] = (v16i){0}; br[j] = (v16i){0}; bi[j] = (v16i){0}; }
+    if constexpr (PAT == 0)
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 2, RS + 1, CS, CS + 2>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+    else if constexpr (PAT == 1)
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 1, RS + 1, CS, CS + 2>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+    else
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS, -1, 0, 0>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+    if constexpr (PAT != 2) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (PAT == 1 && i == 0 && j == 1) continue;
+                const int I = Ib + i, J = Jb + j;
+                int32_t* t = acc + (size_t)(I * (I + 1) / 2 + J) * 2048;
+                store_tile(i == 0 ? ar[j] : br[j], t, lrow, kg, accumulate);
+                store_tile(i == 0 ? ai[j] : bi[j], t + 1024, lrow, kg, accumulate);
+            }
+    }
+}
+
+template <int V>
+struct BlkLds {
+    static constexpr int BYTES = StageOf<V, 12>::KB * 12 * 32 * 64;
+};
+
+// One super-block (I0, J0) of the BLOCKED correlator on 512 threads with local ids ltid.
+template <int NB, int V>
+__device__ __forceinline__ void blk_item(const int8_t* src, int32_t* a, uint8_t* lds0,
+                                         uint8_t* lds1, int n_kb, int accumulate, int I0, int J0,
+                                         int ltid) {
+    constexpr int NE = NB * 32;
+    const size_t slab = (size_t)NE * 64;
+    auto gblk = [=](int s) {
+        const int g = s < 4 ? I0 + s : J0 + (s - 4);
+        return g < NB ? g : NB - 1;  // columns past the array only feed EMPTY waves
+    };
+    const int w = ltid >> 6;
+    static_for<0, 8>([&](auto Wc) {
+        constexpr int W = decltype(Wc)::value;
+        constexpr int RS = 2 * (W / 4), CS = 4 + 2 * (W % 4);
+        if (w == W) {
+            const int Ib = I0 + RS, Jb = J0 + CS - 4;
+            if (Jb + 1 < Ib)
+                blk_prog<V, 0, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+            else if (Jb == Ib)
+                blk_prog<V, 1, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+            else
+                blk_prog<V, 2, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+        }
+    });
+}
+
+template <int NB, int V>
+__global__ __launch_bounds__(512)
+void k_xcorr_blk(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc,
+                 const int2* __restrict__ sblocks, int n_sb, int C, int n_tb, int n_slots,
+                 int accumulate) {
+    constexpr int NE = NB * 32, NT = NB * (NB + 1) / 2;
+    __shared__ __attribute__((aligned(16))) uint8_t lds0[BlkLds<V>::BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds1[BlkLds<V>::BYTES];
+    const int total = n_sb * C * n_slots;
+    const int b = blockIdx.x;
+    const int per_xcd = (total + 7) / 8;
+    // the grid is padded to 8 * per_xcd, so b -> (b % 8) * per_xcd + b / 8 is a bijection
+    const int logical = (b % 8) * per_xcd + b / 8;
+    if (logical >= total) return;
+    const int slot = logical / (n_sb * C);
+    const int rem = logical % (n_sb * C);
+    const int c = rem / n_sb;
+    const int2 sb = sblocks[rem % n_sb];
+    const int kb0 = (int)((long)n_tb * slot / n_slots);
+    const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
+    if (kb1 <= kb0) return;
+    const size_t slab = (size_t)NE * 64;
+    const int8_t* src = tiles + ((size_t)c * n_tb + kb0) * slab;
+    int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
+    blk_item<NB, V>(src, a, lds0, lds1, kb1 - kb0, accumulate, sb.x, sb.y, threadIdx.x);
+}
+
+// ------------------------------------------------------------------------------------------
+// Launch helpers, instantiated per variant in corr_{fold,blk}_pf{1,2,4}.hip
+// ------------------------------------------------------------------------------------------
+template <int NB, int V>
+inline hipError_t launch_fold_t(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
+                                int32_t* acc, int C, int n_tb, int accumulate) {
+    hipLaunchKernelGGL((k_xcorr_fold<NB, V>), dim3(C, p.n_slots), dim3(FoldLayout<NB>::NTH), 0, s,
+                       tiles, acc, C, n_tb, p.n_slots, accumulate);
+    return hipGetLastError();
+}
+
+template <int NB, int V>
+inline hipError_t launch_blk_t(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
+                               int32_t* acc, int C, int n_tb, int accumulate) {
+    const int wgs = p.n_tasks * C * p.n_slots;
+    hipLaunchKernelGGL((k_xcorr_blk<NB, V>), dim3((wgs + 7) / 8 * 8), dim3(512), 0, s, tiles, acc,
+                       (const int2*)p.d_tasks, p.n_tasks, C, n_tb, p.n_slots, accumulate);
+    return hipGetLastError();
+}
+
+template <int V>
+inline hipError_t launch_fold_pf(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
+                                 int32_t* acc, int C, int n_tb, int accumulate) {
+    switch (p.n_elem / 32) {
+    case 1: return launch_fold_t<1, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 2: return launch_fold_t<2, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 3: return launch_fold_t<3, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 4: return launch_fold_t<4, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 5: return launch_fold_t<5, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 6: return launch_fold_t<6, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 7: return launch_fold_t<7, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 8: return launch_fold_t<8, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int V>
+inline hipError_t launch_blk_pf(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
+                                int32_t* acc, int C, int n_tb, int accumulate) {
+    switch (p.n_elem / 32) {
+    case 12: return launch_blk_t<12, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 16: return launch_blk_t<16, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 20: return launch_blk_t<20, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 24: return launch_blk_t<24, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 28: return launch_blk_t<28, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 32: return launch_blk_t<32, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mixer

@@ -1,0 +1,134 @@
+// device_common.h -- device helpers shared by the gfx950 kernels (int8 SWAR, MFMA wrapper,
+// LDS slab staging, accumulator tile store).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+namespace mixer {
+
+
+// Per-byte two's-complement negation of 4 packed int8 (wraps -128 to -128 like the
+// reference's (int8)(-im), src/rx_host_corr.cu:71 and src/cgemm.cuh:108).
+__device__ __forceinline__ int neg8x4(int w) {
+    unsigned nw = ~(unsigned)w;
+    return (int)(((nw & 0x7f7f7f7fu) + 0x01010101u) ^ (nw & 0x80808080u));
+}
+__device__ __forceinline__ v4i neg8x16(v4i v) {
+    v4i r;
+    r.x = neg8x4(v.x); r.y = neg8x4(v.y); r.z = neg8x4(v.z); r.w = neg8x4(v.w);
+    return r;
+}
+// Nonzero iff some byte of w equals 0x80 (-128).
+__device__ __forceinline__ unsigned has80(int w) {
+    unsigned t = (unsigned)w ^ 0x80808080u;
+    return (t - 0x01010101u) & ~t & 0x80808080u;
+}
+
+// MFMA lane layout (checked on gfx950 hardware with a single-instruction probe):
+//   v_mfma_i32_32x32x32_i8: A lane l holds row l%32, k = 16*(l/32)..+15 (16 bytes);
+//   B lane l holds column l%32 with the same k; D lane l register r holds
+//   row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+// Because A and B share the k mapping, loading both operands from the same element x time
+// tile gives sum_k x_i[k] y_j[k] whatever the hardware's k order inside a lane.
+__device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------------------------------
+// Correlator building blocks. A (channel, 32-sample block) slab of NE elements is staged into
+// LDS as [plane][element][32 samples] in 16-byte units,
+// unit(p, e, g) = p*2*NE + 2e + (g ^ bit2(e) ^ bit3(e)): in a fragment read (lane l: element
+// l%32, k-half l/32) every 8 consecutive lanes hit 8 distinct 16-byte bank groups of a 128-byte
+// row and every 16 consecutive lanes 16 distinct groups of a 256-byte row, and the staging
+// writes (64 consecutive units per wave) stay distinct.
+// ------------------------------------------------------------------------------------------
+template <int I, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
+
+template <int NE>
+__device__ __forceinline__ int lds_unit(int p, int e, int g) {
+    return p * 2 * NE + 2 * e + (g ^ (((e >> 2) ^ (e >> 3)) & 1));
+}
+
+// Fragment of element block B (32 elements x 16 samples per lane) of plane p from LDS.
+template <int NE>
+__device__ __forceinline__ v4i frag(const uint8_t* lds, int p, int B, int lrow, int kg) {
+    return *(const v4i*)(lds + (lds_unit<NE>(p, B * 32 + lrow, kg) << 4));
+}
+
+// Copies a list of element blocks of one slab global -> registers -> LDS. NBLK blocks, each
+// 2 planes x 32 elements x 32 samples = 128 chunks of 16 B; NT threads with local ids ltid.
+template <int NE, int NBLK, int NT>
+struct Stager {
+    static constexpr int CHUNKS = NBLK * 128;
+    static constexpr int PER = (CHUNKS + NT - 1) / NT;
+    v4i r[PER];
+    // blk(i) gives the element block of list entry i
+    template <typename BF>
+    __device__ __forceinline__ void load(const int8_t* slab, BF blk, int ltid) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int u = ltid + i * NT;
+            if (CHUNKS % NT == 0 || u < CHUNKS) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                r[i] = *(const v4i*)(slab + (size_t)p * NE * 32 + blk(b) * 1024 + eg * 16);
+            }
+        }
+    }
+    template <typename BF>
+    __device__ __forceinline__ void store(uint8_t* lds, BF blk, int ltid) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int u = ltid + i * NT;
+            if (CHUNKS % NT == 0 || u < CHUNKS) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                const int e = blk(b) * 32 + (eg >> 1), g = eg & 1;
+                *(v4i*)(lds + (lds_unit<NE>(p, e, g) << 4)) = r[i];
+            }
+        }
+    }
+};
+
+// Store one 32x32 int32 accumulator tile from MFMA registers to [32][32] row-major memory.
+__device__ __forceinline__ void store_tile(const v16i& x, int32_t* __restrict__ dst, int lrow,
+                                           int kg, int accumulate) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * kg + (r & 3);
+        int32_t* p = dst + row * 32 + lrow;
+        int v = x[r];
+        if (accumulate) v += *p;
+        *p = v;
+    }
+}
+
+// The four MFMAs of one tile and one 32-sample step, in the stacked real form of the
+// reference's four real products (cgemm.cuh:115-118):
+//   Re += re_I . re_J + wrap(-im_I) . wrap(-im_J)      Im += re_I . wrap(-im_J) + im_I . re_J
+struct Frag { v4i re, im, ng; };
+__device__ __forceinline__ void tile_step(const Frag& a, const Frag& b, v16i& vr, v16i& vi) {
+    vr = mfma(a.re, b.re, vr);
+    vi = mfma(a.re, b.ng, vi);
+    vr = mfma(a.ng, b.ng, vr);
+    vi = mfma(a.im, b.re, vi);
+}
+template <int NE>
+__device__ __forceinline__ Frag load_frag(const uint8_t* lds, int B, int lrow, int kg) {
+    Frag f;
+    f.re = frag<NE>(lds, 0, B, lrow, kg);
+    f.im = frag<NE>(lds, 1, B, lrow, kg);
+    f.ng = neg8x16(f.im);
+    return f;
+}
+
+}  // namespace mixer

@@ -67,6 +67,7 @@ struct mixer_ctx {
     int8_t* d_tiles = nullptr;
     int32_t* d_acc = nullptr;
     uint32_t* d_wrap = nullptr;
+    unsigned long long* d_G = nullptr;  // int64 image grid [C][ng*ng][2], cleared by the image kernel
     float* d_v[2] = {nullptr, nullptr};
     float* d_vcum = nullptr;
     float* d_img[2] = {nullptr, nullptr};  // [C][ng][ng] then vmax[C], vmin[C]
@@ -166,15 +167,15 @@ int finish_integration(mixer_ctx* x) {
     if (x->d_vcum) MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pub_done_cum, 0));
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0));
-    MIXER_HIP(launch_integrate(x->s_comp, x->d_acc, x->d_v[b], x->d_vcum, x->cfg.n_elem,
-                               x->cfg.n_chan, x->n_slots));
+    const bool image = !(x->cfg.flags & MIXER_CFG_NO_IMAGE);
+    MIXER_HIP(launch_integrate(x->s_comp, x->d_acc, x->d_v[b], x->d_vcum, image ? x->d_G : nullptr,
+                               x->cfg.n_elem, x->cfg.n_chan, x->n_slots, x->cfg.grid_x, x->cfg.ng));
     MIXER_TRY(time_end(x, 2, t0));
-    if (!(x->cfg.flags & MIXER_CFG_NO_IMAGE)) {
+    if (image) {
         MIXER_TRY(time_begin(x, &t0));
         float* img = x->d_img[b];
-        MIXER_HIP(launch_image(x->s_comp, x->d_acc, x->d_tw, img, img + x->img_floats,
-                               img + x->img_floats + x->cfg.n_chan, x->cfg.n_elem, x->cfg.n_chan,
-                               x->n_slots, x->cfg.grid_x, x->cfg.ng));
+        MIXER_HIP(launch_image(x->s_comp, x->d_G, x->d_tw, img, img + x->img_floats,
+                               img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng));
         MIXER_TRY(time_end(x, 3, t0));
     }
     MIXER_HIP(hipEventRecord(x->integ_ev[b], x->s_comp));
@@ -193,7 +194,7 @@ int run_update(mixer_ctx* x, const uint8_t* src) {
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0));
     MIXER_HIP(launch_corner_turn(x->s_comp, src, x->d_tiles, x->d_wrap, x->cfg.n_elem,
-                                 x->cfg.n_chan, x->n_tb_upd, 0, x->cfg.n_chan));
+                                 x->cfg.n_chan, x->n_tb_upd, 0, x->cfg.n_chan, x->plan.ct_ch));
     MIXER_TRY(time_end(x, 0, t0));
     return MIXER_OK;
 }
@@ -260,6 +261,7 @@ void destroy_ctx(mixer_ctx* x) {
     if (x->d_tiles) (void)hipFree(x->d_tiles);
     if (x->d_acc) (void)hipFree(x->d_acc);
     if (x->d_wrap) (void)hipFree(x->d_wrap);
+    if (x->d_G) (void)hipFree(x->d_G);
     if (x->s_comp) (void)hipStreamDestroy(x->s_comp);
     if (x->s_copy) (void)hipStreamDestroy(x->s_copy);
     if (x->s_pub) (void)hipStreamDestroy(x->s_pub);
@@ -272,6 +274,10 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MIXER_ERR_NODEV;
     MIXER_HIP(hipSetDevice(device));
+    if ((cfg->flags & MIXER_CFG_AUTOTUNE) && !has_tuning(cfg->n_elem)) {
+        mixer_tuning best;
+        MIXER_TRY(mixer_autotune(cfg, device, &best));
+    }
     MIXER_TRY(get_plan(*cfg, &x->plan));
     x->n_slots = x->plan.n_slots;
     x->n_tb_upd = cfg->k_update / cfg->t_pkt;
@@ -303,6 +309,9 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
     MIXER_HIP(hipMalloc(&x->d_acc, mixer_acc_bytes(cfg)));
     MIXER_HIP(hipMalloc(&x->d_wrap, sizeof(uint32_t)));
     MIXER_HIP(hipMemset(x->d_wrap, 0, sizeof(uint32_t)));
+    const size_t gbytes = (size_t)cfg->n_chan * cfg->ng * cfg->ng * 16;
+    MIXER_HIP(hipMalloc(&x->d_G, gbytes));
+    MIXER_HIP(hipMemset(x->d_G, 0, gbytes));
     MIXER_TRY(get_twiddles(cfg->ng, &x->d_tw));
     MIXER_HIP(hipDeviceSynchronize());
     return MIXER_OK;

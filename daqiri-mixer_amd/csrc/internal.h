@@ -16,6 +16,8 @@ std::vector<std::pair<int, int>> sblocks_for(int nb);
 int get_plan(const mixer_array_cfg& c, XcorrPlan* p);
 int get_twiddles(int ng, const float2** out);
 int current_device();
+mixer_tuning tuning_for(int n_elem);
+bool has_tuning(int n_elem);
 
 #define MIXER_TRY(x)                  \
     do {                              \

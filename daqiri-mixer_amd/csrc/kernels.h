@@ -8,29 +8,62 @@ namespace mixer {
 // How the correlator covers the lower triangle of one channel.
 //   FOLD     arrays of <= 8 blocks of 32 elements: one workgroup per (channel, slot), one wave
 //            per folded row pair (r, nb-1-r); balanced (nb+1 tiles per wave).
-//   BLOCKED  larger arrays: one wave per 2x4 tile block, four per workgroup, an explicit task
-//            list (I0, J0) on the device.
+//   BLOCKED  larger arrays: one workgroup per 4x8-tile super-block, one wave per 2x2 tile
+//            block (8 waves), an explicit super-block list (I0, J0) on the device.
 // n_slots splits the time axis of one call into partial int32 sums (summed exactly by the
 // integrate and image kernels) so that small channel counts still fill all 256 CUs.
 struct XcorrPlan {
     enum Kind { FOLD = 0, BLOCKED = 1 } kind;
     int n_elem;
     int n_slots;
+    int prefetch;         // register prefetch stages of the time loop: 1, 2 or 4
+    int ct_ch;            // corner-turn channels per workgroup: 8, 16 or 32
     int n_tasks;          // BLOCKED only
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
 };
 
 hipError_t launch_corner_turn(hipStream_t s, const uint8_t* heaps, int8_t* tiles, uint32_t* wrap,
-                              int N, int C, int n_tb, int c_base, int c_count);
+                              int N, int C, int n_tb, int c_base, int c_count, int cch);
 hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& plan, const int8_t* tiles, int32_t* acc,
                         int C, int n_tb, int accumulate);
-hipError_t launch_integrate(hipStream_t s, const int32_t* acc, float* v_store, float* v_add, int N,
-                            int C, int n_slots);
+hipError_t launch_integrate(hipStream_t s, const int32_t* acc, float* v_store, float* v_add,
+                            unsigned long long* G, int N, int C, int n_slots, int grid_x, int ng);
 size_t image_lds_bytes(int ng);
 hipError_t set_image_lds_limit(int ng);
-hipError_t launch_image(hipStream_t s, const int32_t* acc, const float2* tw, float* img,
-                        float* vmax, float* vmin, int N, int C, int n_slots, int grid_x, int ng);
+hipError_t launch_image(hipStream_t s, unsigned long long* G, const float2* tw, float* img,
+                        float* vmax, float* vmin, int C, int ng);
 hipError_t launch_xcorr_plain(hipStream_t s, const int8_t* tiles, int32_t* vre, int32_t* vim, int N,
                               int n_tb, int c);
+
+hipError_t launch_fill_random(hipStream_t s, uint8_t* p, size_t bytes, uint64_t seed);
+
+// One array ("job") of a batch, as the batched kernels see it (device memory).
+struct BatchJobDev {
+    const uint8_t* heaps;        // [n_tb][n_elem][payload]
+    int8_t* tiles;               // [n_chan][n_tb][2][n_elem][32]
+    int32_t* acc;                // [n_slots][n_chan][tiles][2][32][32]
+    float* v;                    // [n_chan][2][n_elem][n_elem]
+    unsigned long long* G;       // [n_chan][ng*ng][2]
+    float* img;                  // [n_chan][ng][ng], then vmax[n_chan], vmin[n_chan]
+    const float2* tw;            // FFT twiddles for ng
+    const int2* sblocks;         // BLOCKED super-blocks (I0, J0)
+    uint32_t* wrap;              // corner-turn -128 counter
+    int n_elem, n_chan, n_tb, n_slots, grid_x, ng, nb, n_sb;
+    int ct_wg0, int_wg0, img_wg0;  // first workgroup of this job in each batched launch
+    int ch_per_item;               // FOLD: channels one correlator work item covers (>= 1)
+};
+
+// Correlator work item: (job, first channel, super-block index or 0, time slot)
+hipError_t launch_batch_reset(hipStream_t s, int* counter);
+hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
+hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
+                              int n_items, int* counter, int n_wgs);
+hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
+                                  int max_ng);
+hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
+                              int max_ng);
+hipError_t set_batch_lds_limits(int max_ng);
+// channels packed into one correlator workgroup for an array of nb blocks (FOLD with <= 5 blocks)
+int batch_fold_groups(int nb);
 
 }  // namespace mixer

@@ -1,0 +1,307 @@
+// kernels_batch.hip -- batched launches over many independent arrays ("jobs") of different
+// sizes: one corner-turn launch, one persistent correlator launch pulling work items from an
+// atomic counter, one integrate (+ gridding) launch and one image launch. The batch engine
+// captures the sequence once as a hipGraph and replays it.
+#include "corr_impl.h"
+#include "kernels_body.h"
+
+namespace mixer {
+
+// Job j owning global workgroup b of a batched launch, from the per-job first-workgroup
+// offsets (jobs are few, a linear scan over the table is cheap).
+template <typename F>
+__device__ __forceinline__ int job_of(const BatchJobDev* jobs, int n_jobs, int b, F wg0) {
+    int j = 0;
+    while (j + 1 < n_jobs && b >= wg0(jobs[j + 1])) ++j;
+    return j;
+}
+
+__global__ void k_batch_reset(int* counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+constexpr int kBatchCCH = 8;  // corner-turn channels per workgroup in batched launches
+
+__global__ __launch_bounds__(256) void k_batch_corner_turn(const BatchJobDev* __restrict__ jobs,
+                                                           int n_jobs) {
+    const int b = blockIdx.x;
+    const int j = job_of(jobs, n_jobs, b, [](const BatchJobDev& J) { return J.ct_wg0; });
+    const BatchJobDev& J = jobs[j];
+    const int local = b - J.ct_wg0;
+    const int tb = local % J.n_tb, r = local / J.n_tb;
+    const int eb = r % J.nb, cc = r / J.nb;
+    corner_turn_body<kBatchCCH>(J.heaps, J.tiles, J.wrap, J.n_elem, J.n_chan, J.n_tb, tb, eb * 32,
+                                cc * kBatchCCH, J.n_chan);
+}
+
+// ------------------------------------------------------------------------------------------
+// Runtime-generic correlator wave program for the persistent kernel: one code path for every
+// array size. A wave owns up to kMaxT tiles: row A (LDS block ra, global block ra_g) with
+// columns [a0, a0+na) and row B with columns [b0, b0+nb) (LDS slots; global *_g), accumulator
+// slot s < na for row A and na <= s < na+nb for row B, so accumulator indices stay static.
+// The group's nth threads (local ids ltid; ltid < 0 for waves outside any group, which only
+// meet the barriers) stage nblk element blocks (global block gblk(b)) per 32-sample block
+// through registers into two LDS buffers (one barrier per block, like corr_wave with V = 1).
+// ------------------------------------------------------------------------------------------
+constexpr int kMaxT = 5;
+constexpr int kMaxStage = 3;  // 16-byte chunks staged per thread per block (NBLK*128 <= 3*512)
+
+struct WaveTiles {
+    int ra, ra_g, a0, a0_g, na;
+    int rb, rb_g, b0, b0_g, nb;
+};
+
+__device__ __forceinline__ int lds_unit_rt(int ne, int p, int e, int g) {
+    return p * 2 * ne + 2 * e + (g ^ (((e >> 2) ^ (e >> 3)) & 1));
+}
+__device__ __forceinline__ Frag load_frag_rt(const uint8_t* L, int ne, int B, int lrow, int kg) {
+    Frag f;
+    f.re = *(const v4i*)(L + (lds_unit_rt(ne, 0, B * 32 + lrow, kg) << 4));
+    f.im = *(const v4i*)(L + (lds_unit_rt(ne, 1, B * 32 + lrow, kg) << 4));
+    f.ng = neg8x16(f.im);
+    return f;
+}
+
+template <typename GB>
+__device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, size_t slab, int nblk,
+                                             GB gblk, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                             const WaveTiles& wt, bool active, int32_t* __restrict__ acc,
+                                             int ltid, int nth) {
+    const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
+    const int ne = nblk * 32, chunks = nblk * 128;
+    v16i ar[kMaxT], ai[kMaxT];
+#pragma unroll
+    for (int s = 0; s < kMaxT; ++s) { ar[s] = (v16i){0}; ai[s] = (v16i){0}; }
+    v4i st[kMaxStage];
+    auto gload = [&](const int8_t* sl) {
+#pragma unroll
+        for (int i = 0; i < kMaxStage; ++i) {
+            const int u = ltid + i * nth;
+            if (ltid >= 0 && u < chunks) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                st[i] = *(const v4i*)(sl + (size_t)p * (slab >> 1) + gblk(b) * 1024 + eg * 16);
+            }
+        }
+    };
+    auto sstore = [&](uint8_t* L) {
+#pragma unroll
+        for (int i = 0; i < kMaxStage; ++i) {
+            const int u = ltid + i * nth;
+            if (ltid >= 0 && u < chunks) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                *(v4i*)(L + (lds_unit_rt(ne, p, b * 32 + (eg >> 1), eg & 1) << 4)) = st[i];
+            }
+        }
+    };
+    gload(src);
+    sstore(lds0);
+    __syncthreads();
+    for (int kb = 0; kb < n_kb; ++kb) {
+        const bool more = kb + 1 < n_kb;
+        if (more) gload(src + (size_t)(kb + 1) * slab);
+        if (active) {
+            const uint8_t* L = (kb & 1) ? lds1 : lds0;
+            const Frag fa = load_frag_rt(L, ne, wt.ra, lrow, kg);
+            const Frag fb = wt.nb ? load_frag_rt(L, ne, wt.rb, lrow, kg) : fa;
+#pragma unroll
+            for (int s = 0; s < kMaxT; ++s) {
+                if (s < wt.na) {
+                    const Frag b = load_frag_rt(L, ne, wt.a0 + s, lrow, kg);
+                    tile_step(fa, b, ar[s], ai[s]);
+                } else if (s < wt.na + wt.nb) {
+                    const Frag b = load_frag_rt(L, ne, wt.b0 + s - wt.na, lrow, kg);
+                    tile_step(fb, b, ar[s], ai[s]);
+                }
+            }
+        }
+        if (more) sstore((kb & 1) ? lds0 : lds1);
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int s = 0; s < kMaxT; ++s) {
+        if (s < wt.na + wt.nb) {
+            const int I = s < wt.na ? wt.ra_g : wt.rb_g;
+            const int J = s < wt.na ? wt.a0_g + s : wt.b0_g + s - wt.na;
+            int32_t* t = acc + (size_t)(I * (I + 1) / 2 + J) * 2048;
+            store_tile(ar[s], t, lrow, kg, 0);
+            store_tile(ai[s], t + 1024, lrow, kg, 0);
+        }
+    }
+}
+
+// Waves per channel of the folded layout in the persistent kernel: NB = 1 is one tile on one
+// wave; NB >= 2 splits each row pair (p, NB-1-p) over two waves (at most 5 tiles per wave).
+__device__ __host__ inline int fold_waves_rt(int nb) { return nb == 1 ? 1 : 2 * ((nb + 1) / 2); }
+
+int batch_fold_groups(int nb) { return nb > 8 ? 1 : 8 / fold_waves_rt(nb); }
+
+// The split-fold tiles of wave w of an nb-block channel (same layout as FoldLayout<NB, 2>).
+__device__ __forceinline__ WaveTiles fold_tiles_rt(int nb, int w, bool* active) {
+    WaveTiles t{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    *active = w < fold_waves_rt(nb);
+    if (!*active) return t;
+    if (nb == 1) {
+        t.na = 1;
+        return t;
+    }
+    const int P = (nb + 1) / 2, p = w % P, R1 = p, R2 = nb - 1 - p;
+    const int T = (R1 == R2) ? R1 + 1 : R1 + R2 + 2, S = (T + 1) / 2;
+    if (w < P) {
+        t.ra = t.ra_g = R2; t.a0 = t.a0_g = 0; t.na = S;
+    } else if (R1 == R2) {
+        t.ra = t.ra_g = R2; t.a0 = t.a0_g = S; t.na = R2 + 1 - S;
+    } else {
+        t.ra = t.ra_g = R1; t.a0 = t.a0_g = 0; t.na = R1 + 1;
+        if (S <= R2) { t.rb = t.rb_g = R2; t.b0 = t.b0_g = S; t.nb = R2 + 1 - S; }
+    }
+    *active = t.na + t.nb > 0;
+    return t;
+}
+
+// The 2x2 tile block of wave w in BLOCKED super-block (I0, J0): LDS slots 0..3 hold the rows
+// I0..I0+3, slots 4..11 the columns J0..J0+7.
+__device__ __forceinline__ WaveTiles blk_tiles_rt(int I0, int J0, int w, bool* active) {
+    WaveTiles t{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int RS = 2 * (w / 4), CS = 2 * (w % 4);
+    const int Ib = I0 + RS, Jb = J0 + CS;
+    *active = Jb <= Ib;
+    if (!*active) return t;
+    const bool full = Jb + 1 < Ib;
+    t.ra = RS; t.ra_g = Ib; t.a0 = 4 + CS; t.a0_g = Jb; t.na = full ? 2 : 1;
+    t.rb = RS + 1; t.rb_g = Ib + 1; t.b0 = 4 + CS; t.b0_g = Jb; t.nb = 2;
+    return t;
+}
+
+// LDS of the persistent correlator: two stage buffers per channel group; packed folds need at
+// most 32 KiB (e.g. 4 groups x 2 x 4 KiB at 64 elements), BLOCKED 2 x 24 KiB.
+constexpr int kBatchLds = 48 * 1024;
+
+// Persistent correlator: each workgroup takes the next work item (job, first channel,
+// super-block, time slot) until the list is exhausted. Items are ordered by decreasing cost,
+// so the longest start first and the tail is made of short items.
+__global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restrict__ jobs,
+                                                     const int4* __restrict__ items, int n_items,
+                                                     int* __restrict__ counter) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kBatchLds];
+    __shared__ int item_sh;
+    const int w8 = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item_sh;
+        __syncthreads();
+        if (it >= n_items) break;  // every wave reaches this with the same value
+        const int4 w = items[it];
+        const BatchJobDev& J = jobs[w.x];
+        const int nb = J.nb, slot = w.w;
+        const int kb0 = (int)((long)J.n_tb * slot / J.n_slots);
+        const int kb1 = (int)((long)J.n_tb * (slot + 1) / J.n_slots);
+        const int NT = nb * (nb + 1) / 2;
+        if (kb1 > kb0) {  // uniform
+            // One call site for every array size (a second inlined copy makes the allocator
+            // spill): fold channels and BLOCKED super-blocks differ only in these parameters.
+            const size_t slab = (size_t)nb * 32 * 64;
+            int nblk, buf, ltid, nth, c, row0, col0;
+            bool active = false, have = true;
+            WaveTiles wt;
+            uint8_t* l0 = lds;
+            if (nb <= 8) {
+                const int wpc = fold_waves_rt(nb), G = min(8 / wpc, J.ch_per_item);
+                const int g = w8 / wpc;
+                nblk = nb;
+                buf = nb * 32 * 64;
+                nth = 64 * wpc;
+                ltid = g < G ? (int)(threadIdx.x % nth) : -1;
+                c = w.y + (g < G ? g : 0);
+                have = g < G && c < J.n_chan;
+                if (!have) c = w.y;  // idle groups stage a valid slab and store nothing
+                wt = fold_tiles_rt(nb, w8 % wpc, &active);
+                l0 = lds + (g < G ? g : 0) * 2 * buf;
+                row0 = 0;
+                col0 = 4;  // identity block map
+            } else {
+                const int2 sb = J.sblocks[w.z];
+                nblk = 12;
+                buf = 12 * 32 * 64;
+                nth = 512;
+                ltid = threadIdx.x;
+                c = w.y;
+                wt = blk_tiles_rt(sb.x, sb.y, w8, &active);
+                row0 = sb.x;
+                col0 = sb.y;
+            }
+            auto gblk = [=](int b) {
+                const int g = b < 4 ? row0 + b : col0 + (b - 4);
+                return g < nb ? g : nb - 1;  // columns past the array only feed idle waves
+            };
+            const int8_t* src = J.tiles + ((size_t)c * J.n_tb + kb0) * slab;
+            int32_t* a = J.acc + ((size_t)slot * J.n_chan + c) * NT * 2048;
+            corr_wave_rt(src, slab, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
+                         ltid, nth);
+        }
+        __syncthreads();  // LDS reuse by the next item
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_integrate(const BatchJobDev* __restrict__ jobs,
+                                                         int n_jobs) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int b = blockIdx.x;
+    const int j = job_of(jobs, n_jobs, b, [](const BatchJobDev& J) { return J.int_wg0; });
+    const BatchJobDev& J = jobs[j];
+    const int local = b - J.int_wg0;
+    integrate_body(J.acc, J.v, nullptr, J.G, J.n_elem, J.n_chan, J.n_slots, J.grid_x, J.ng,
+                   local % J.nb, local / J.nb, smem);
+}
+
+__global__ __launch_bounds__(256) void k_batch_image(const BatchJobDev* __restrict__ jobs,
+                                                     int n_jobs) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int b = blockIdx.x;
+    const int j = job_of(jobs, n_jobs, b, [](const BatchJobDev& J) { return J.img_wg0; });
+    const BatchJobDev& J = jobs[j];
+    const size_t nimg = (size_t)J.n_chan * J.ng * J.ng;
+    image_body(J.G, J.tw, J.img, J.img + nimg, J.img + nimg + J.n_chan, J.ng, b - J.img_wg0, smem);
+}
+
+hipError_t launch_batch_reset(hipStream_t s, int* counter) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs) {
+    hipLaunchKernelGGL(k_batch_corner_turn, dim3(n_wgs), dim3(256), 0, s, jobs, n_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
+                              int n_items, int* counter, int n_wgs) {
+    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
+                                  int max_ng) {
+    hipLaunchKernelGGL(k_batch_integrate, dim3(n_wgs), dim3(256), (size_t)max_ng * max_ng * 16, s,
+                       jobs, n_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
+                              int max_ng) {
+    hipLaunchKernelGGL(k_batch_image, dim3(n_wgs), dim3(256), (size_t)max_ng * max_ng * 16, s, jobs,
+                       n_jobs);
+    return hipGetLastError();
+}
+
+hipError_t set_batch_lds_limits(int max_ng) {
+    const int bytes = max_ng * max_ng * 16;
+    hipError_t e = hipFuncSetAttribute((const void*)k_batch_integrate,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_batch_image, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               bytes);
+}
+
+}  // namespace mixer

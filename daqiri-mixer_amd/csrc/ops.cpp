@@ -24,12 +24,31 @@ int hip_status(hipError_t e) {
     return MIXER_ERR_HIP;
 }
 
+namespace {
+std::mutex g_tune_mtx;
+std::map<int, mixer_tuning> g_tuning;  // n_elem -> tuning (0 fields = automatic)
+}  // namespace
+
+mixer_tuning tuning_for(int n_elem) {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    auto it = g_tuning.find(n_elem);
+    if (it == g_tuning.end()) return mixer_tuning{0, 0, 0, 0};
+    return it->second;
+}
+
+bool has_tuning(int n_elem) {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    return g_tuning.count(n_elem) != 0;
+}
+
 // Time slots: split each channel's time axis so the correlator grid reaches ~2 workgroups per
 // CU-equivalent of work (256 CUs). Depends on the geometry only, so every call on the same cfg
 // (correlate, integrate, image) agrees on the accumulator layout.
 int plan_slots(const mixer_array_cfg& c) {
     const int nb = c.n_elem / 32;
     const int n_tb = c.k_update / c.t_pkt;
+    const mixer_tuning t = tuning_for(c.n_elem);
+    if (t.slots > 0) return t.slots < n_tb ? t.slots : n_tb;
     long units;  // workgroups per slot
     if (nb <= 8) {
         units = c.n_chan;
@@ -70,6 +89,9 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     const int nb = c.n_elem / 32;
     p->n_elem = c.n_elem;
     p->n_slots = plan_slots(c);
+    const mixer_tuning t = tuning_for(c.n_elem);
+    p->prefetch = t.prefetch > 0 ? t.prefetch : 1;
+    p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
     p->n_tasks = 0;
     p->d_tasks = nullptr;
     if (nb <= 8) {
@@ -151,7 +173,7 @@ int mixer_validate_cfg(const mixer_array_cfg* c) {
     const int N = c->n_elem, nb = N / 32;
     if (N < 32 || N > 1024 || N % 32) return MIXER_ERR_ARG;
     if (nb > 8 && nb % 4) return MIXER_ERR_ARG;
-    if (c->grid_x < 1 || N % c->grid_x) return MIXER_ERR_ARG;
+    if (c->grid_x < 1 || c->grid_x > 32 || 32 % c->grid_x) return MIXER_ERR_ARG;  // rows of 32-element tiles
     if (c->n_chan < 1 || c->n_chan > 1024) return MIXER_ERR_ARG;
     if (c->t_pkt != 32) return MIXER_ERR_ARG;
     if (c->seq_byte < 0 || c->hdr_bytes < c->seq_byte + 4) return MIXER_ERR_ARG;
@@ -206,8 +228,10 @@ int mixer_corner_turn(void* stream, const mixer_array_cfg* c, const uint8_t* d_h
     int st = mixer_validate_cfg(c);
     if (st) return st;
     if (!d_heaps || !d_tiles || n_tb <= 0) return MIXER_ERR_ARG;
+    XcorrPlan p;
+    if ((st = get_plan(*c, &p))) return st;
     return hip_status(launch_corner_turn((hipStream_t)stream, d_heaps, d_tiles, d_wrap, c->n_elem,
-                                         c->n_chan, n_tb, 0, c->n_chan));
+                                         c->n_chan, n_tb, 0, c->n_chan, p.ct_ch));
 }
 
 int mixer_acc_slots(const mixer_array_cfg* c) {
@@ -239,10 +263,12 @@ int mixer_integrate(void* stream, const mixer_array_cfg* c, const int32_t* d_acc
     if (st) return st;
     if (!d_acc || !d_v) return MIXER_ERR_ARG;
     return hip_status(launch_integrate((hipStream_t)stream, d_acc, accumulate ? nullptr : d_v,
-                                       accumulate ? d_v : nullptr, c->n_elem, c->n_chan,
-                                       plan_slots(*c)));
+                                       accumulate ? d_v : nullptr, nullptr, c->n_elem, c->n_chan,
+                                       plan_slots(*c), c->grid_x, c->ng));
 }
 
+// Gridding runs as the grid-only form of the integrate kernel into a stream-ordered int64
+// workspace, then the per-channel FFT kernel reads (and clears) it.
 int mixer_image(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, float* d_img,
                 float* d_vmax, float* d_vmin) {
     int st = mixer_validate_cfg(c);
@@ -250,8 +276,42 @@ int mixer_image(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, fl
     if (!d_acc || !d_img || !d_vmax || !d_vmin) return MIXER_ERR_ARG;
     const float2* tw = nullptr;
     if ((st = get_twiddles(c->ng, &tw))) return st;
-    return hip_status(launch_image((hipStream_t)stream, d_acc, tw, d_img, d_vmax, d_vmin, c->n_elem,
-                                   c->n_chan, plan_slots(*c), c->grid_x, c->ng));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t gbytes = (size_t)c->n_chan * c->ng * c->ng * 16;
+    void* g = nullptr;
+    if ((st = hip_status(hipMallocAsync(&g, gbytes, s)))) return st;
+    st = hip_status(hipMemsetAsync(g, 0, gbytes, s));
+    if (!st)
+        st = hip_status(launch_integrate(s, d_acc, nullptr, nullptr, (unsigned long long*)g, c->n_elem,
+                                         c->n_chan, plan_slots(*c), c->grid_x, c->ng));
+    if (!st)
+        st = hip_status(launch_image(s, (unsigned long long*)g, tw, d_img, d_vmax, d_vmin, c->n_chan,
+                                     c->ng));
+    const int st2 = hip_status(hipFreeAsync(g, s));
+    return st ? st : st2;
+}
+
+int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
+    if (!t || n_elem < 32 || n_elem > 1024 || n_elem % 32) return MIXER_ERR_ARG;
+    if (t->slots < 0 || t->slots > 8) return MIXER_ERR_ARG;
+    if (t->prefetch != 0 && t->prefetch != 1 && t->prefetch != 2 && t->prefetch != 4) return MIXER_ERR_ARG;
+    if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    g_tuning[n_elem] = *t;
+    return MIXER_OK;
+}
+
+int mixer_get_tuning(const mixer_array_cfg* c, mixer_tuning* t) {
+    int st = mixer_validate_cfg(c);
+    if (st) return st;
+    if (!t) return MIXER_ERR_ARG;
+    const int nb = c->n_elem / 32;
+    t->slots = plan_slots(*c);
+    const mixer_tuning u = tuning_for(c->n_elem);
+    t->prefetch = u.prefetch > 0 ? u.prefetch : 1;
+    t->ct_chan = u.ct_chan > 0 ? u.ct_chan : 8;
+    t->reserved = nb <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
+    return MIXER_OK;
 }
 
 int mixer_xcorr_plain(void* stream, const mixer_array_cfg* c, const uint8_t* d_heaps, int n_tb,
@@ -262,7 +322,7 @@ int mixer_xcorr_plain(void* stream, const mixer_array_cfg* c, const uint8_t* d_h
     hipStream_t s = (hipStream_t)stream;
     for (int ch = 0; ch < c->n_chan; ++ch) {
         if ((st = hip_status(launch_corner_turn(s, d_heaps, d_tiles, nullptr, c->n_elem, c->n_chan,
-                                                n_tb, ch, 1))))
+                                                n_tb, ch, 1, 8))))
             return st;
         if ((st = hip_status(launch_xcorr_plain(s, d_tiles, d_vre, d_vim, c->n_elem, n_tb, ch))))
             return st;

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmixer.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BUDGET, ERR_NODEV, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
-CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP = 1, 2, 4
+CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP, CFG_AUTOTUNE = 1, 2, 4, 8
 H2D, D2H, D2D = 0, 1, 2
 
 
@@ -49,6 +49,26 @@ class Stats(C.Structure):
         ("last_dump_latency_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {"slots": self.slots, "prefetch": self.prefetch, "ct_chan": self.ct_chan}
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("pack", C.c_int32), ("tb_target", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {"pack": self.pack, "tb_target": self.tb_target, "use_graph": self.use_graph}
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("n_jobs", C.c_int32), ("n_items", C.c_int32), ("n_wgs", C.c_int32), ("graph", C.c_int32),
+                ("payload_bytes", C.c_uint64), ("exec_ops", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
 class Sky(C.Structure):
     _fields_ = [("planet_l", C.c_double), ("planet_m", C.c_double), ("sigma", C.c_double),
                 ("star_amp", C.c_double), ("planet_scale", C.c_double), ("nch_gen", C.c_int32),
@@ -75,6 +95,9 @@ SIGNATURES = [
     ("mixer_xcorr", I32, [P, P, P, I32, P, I32]),
     ("mixer_integrate", I32, [P, P, P, P, I32]),
     ("mixer_image", I32, [P, P, P, P, P, P]),
+    ("mixer_set_tuning", I32, [I32, P]),
+    ("mixer_get_tuning", I32, [P, P]),
+    ("mixer_autotune", I32, [P, I32, P]),
     ("mixer_xcorr_plain", I32, [P, P, P, I32, P, P, P]),
     ("mixer_create", I32, [P, I32, P]),
     ("mixer_destroy", I32, [P]),
@@ -93,6 +116,18 @@ SIGNATURES = [
     ("mixer_set_timing", I32, [P, I32]),
     ("mixer_get_timing", I32, [P, I32, P, P]),
     ("mixer_compute_stream", P, [P]),
+    ("mixer_batch_default_tuning", None, [P]),
+    ("mixer_batch_create", I32, [P, I32, I32, P, P]),
+    ("mixer_batch_destroy", I32, [P]),
+    ("mixer_batch_run", I32, [P, P]),
+    ("mixer_batch_sync", I32, [P]),
+    ("mixer_batch_stream", P, [P]),
+    ("mixer_batch_get_info", I32, [P, P]),
+    ("mixer_batch_dump", I32, [P, I32, P]),
+    ("mixer_batch_read_image", I32, [P, I32, P, P, P]),
+    ("mixer_batch_acc", I32, [P, I32, P, P]),
+    ("mixer_batch_autotune", I32, [P, I32, P]),
+    ("mixer_fill_random", I32, [P, P, SZ, U64]),
     ("mixer_set_device", I32, [I32]),
     ("mixer_dev_alloc", I32, [SZ, P]),
     ("mixer_dev_free", I32, [P]),
@@ -247,6 +282,23 @@ def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):
           "mixer_xcorr_plain")
 
 
+def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0):
+    t = Tuning(slots, prefetch, ct_chan, 0)
+    check(lib().mixer_set_tuning(n_elem, C.byref(t)), "mixer_set_tuning")
+
+
+def get_tuning(cfg) -> Tuning:
+    t = Tuning()
+    check(lib().mixer_get_tuning(C.byref(cfg), C.byref(t)), "mixer_get_tuning")
+    return t
+
+
+def autotune(cfg, device: int = 0) -> Tuning:
+    t = Tuning()
+    check(lib().mixer_autotune(C.byref(cfg), device, C.byref(t)), "mixer_autotune")
+    return t
+
+
 def tiles_bytes(cfg, n_tb: int) -> int:
     return int(lib().mixer_tiles_bytes(C.byref(cfg), n_tb))
 
@@ -350,6 +402,81 @@ class Engine:
         n = C.c_uint64(0)
         check(lib().mixer_get_timing(self.h, stage, C.byref(ms), C.byref(n)), "mixer_get_timing")
         return ms.value, n.value
+
+
+# ---- batch of arrays ----------------------------------------------------------------------------
+def batch_default_tuning() -> BatchTuning:
+    t = BatchTuning()
+    lib().mixer_batch_default_tuning(C.byref(t))
+    return t
+
+
+def fill_random(d: DeviceBuffer, seed: int, stream=None, nbytes: int | None = None):
+    """Fill device memory with int8 values uniform in [-127, 127] (keyed by seed and offset)."""
+    check(lib().mixer_fill_random(stream, d.ptr, d.nbytes if nbytes is None else nbytes, seed),
+          "mixer_fill_random")
+
+
+class Batch:
+    """Many independent arrays correlated together: one integration per array per run()."""
+
+    def __init__(self, cfgs, device: int = 0, tuning: BatchTuning | None = None):
+        self.cfgs = list(cfgs)
+        arr = (ArrayCfg * len(self.cfgs))(*self.cfgs)
+        h = C.c_void_p()
+        check(lib().mixer_batch_create(arr, len(self.cfgs), device,
+                                       C.byref(tuning) if tuning is not None else None, C.byref(h)),
+              "mixer_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(lib().mixer_batch_destroy(self.h), "mixer_batch_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, heaps):
+        """heaps: one device pointer (DeviceBuffer or int address) per array, header-free
+        [n_tb][n_elem][payload] layout."""
+        ptrs = (C.c_void_p * len(heaps))(*[h.ptr.value if isinstance(h, DeviceBuffer) else int(h)
+                                          for h in heaps])
+        check(lib().mixer_batch_run(self.h, ptrs), "mixer_batch_run")
+
+    def sync(self):
+        check(lib().mixer_batch_sync(self.h), "mixer_batch_sync")
+
+    def stream(self):
+        return lib().mixer_batch_stream(self.h)
+
+    def info(self) -> BatchInfo:
+        i = BatchInfo()
+        check(lib().mixer_batch_get_info(self.h, C.byref(i)), "mixer_batch_get_info")
+        return i
+
+    def dump(self, job: int) -> np.ndarray:
+        c = self.cfgs[job]
+        out = np.empty((c.n_chan, 2, c.n_elem, c.n_elem), np.float32)
+        check(lib().mixer_batch_dump(self.h, job, ptr(out)), "mixer_batch_dump")
+        return out
+
+    def read_image(self, job: int):
+        c = self.cfgs[job]
+        img = np.empty((c.n_chan, c.ng, c.ng), np.float32)
+        vmax = np.empty(c.n_chan, np.float32)
+        vmin = np.empty(c.n_chan, np.float32)
+        check(lib().mixer_batch_read_image(self.h, job, ptr(img), ptr(vmax), ptr(vmin)),
+              "mixer_batch_read_image")
+        return img, vmax, vmin
+
+    def autotune(self, reps: int = 3) -> BatchTuning:
+        t = BatchTuning()
+        check(lib().mixer_batch_autotune(self.h, reps, C.byref(t)), "mixer_batch_autotune")
+        return t
 
 
 def corr32_bytes(cfg) -> int:

@@ -59,7 +59,7 @@ enum {
  * form one time block of t_pkt samples. */
 typedef struct mixer_array_cfg {
     int32_t n_elem;    /* elements N; a multiple of 32, 32..1024 */
-    int32_t grid_x;    /* elements per grid row; element a sits at (a % grid_x, a / grid_x) */
+    int32_t grid_x;    /* elements per grid row, a divisor of 32; element a sits at (a % grid_x, a / grid_x) */
     int32_t n_chan;    /* channels per packet C, 1..1024 */
     int32_t t_pkt;     /* samples per packet per channel; must be 32 (one MFMA K step) */
     int32_t hdr_bytes; /* header bytes before the payload (64) */
@@ -75,6 +75,8 @@ typedef struct mixer_array_cfg {
 #define MIXER_CFG_NO_IMAGE 2   /* skip gridding + FFT at each integration */
 #define MIXER_CFG_NOFLOP 4     /* ingest only: copy heaps, skip all compute (--noflop,
                                   src/rx_host_corr.cu:161) */
+#define MIXER_CFG_AUTOTUNE 8   /* mixer_create runs mixer_autotune when the size class has no
+                                  tuning yet */
 
 void mixer_default_cfg(mixer_array_cfg* cfg);
 int mixer_validate_cfg(const mixer_array_cfg* cfg);
@@ -125,6 +127,27 @@ int mixer_integrate(void* stream, const mixer_array_cfg* cfg, const int32_t* d_a
  *   d_img [n_chan][ng][ng] f32 = Re F[(y+ng/2)%ng][(x+ng/2)%ng], d_vmax/d_vmin [n_chan]. */
 int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_img,
                 float* d_vmax, float* d_vmin);
+
+/* Launch tuning (the autotuner's search space), per array size class n_elem:
+ *   slots     time split of one correlator call into int32 partial sums (1..8)
+ *   prefetch  register prefetch stages of the correlator's 32-sample time loop (1, 2 or 4)
+ *   ct_chan   corner-turn channels per workgroup (8, 16 or 32)
+ * 0 in a field means automatic. The table is process-wide. slots changes the accumulator
+ * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
+ * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
+ * src/cgemm.cuh:18-24). */
+typedef struct mixer_tuning {
+    int32_t slots;
+    int32_t prefetch;
+    int32_t ct_chan;
+    int32_t reserved;
+} mixer_tuning;
+int mixer_set_tuning(int n_elem, const mixer_tuning* t);
+int mixer_get_tuning(const mixer_array_cfg* cfg, mixer_tuning* t); /* the plan in effect */
+/* Autotuner: times corner-turn + correlate + integrate + image of one integration of cfg on
+ * synthetic device data for every (slots, prefetch) candidate, then every ct_chan, on device
+ * `device`; stores the fastest in the tuning table and returns it in *best. */
+int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuning* best);
 
 /* Plain per-channel-launch form (baseline): one corner-turn launch and one full-square GEMM
  * launch per channel, exactly the reference's 4-real-product math (cgemm.cuh:114-119) on the
@@ -198,6 +221,58 @@ int mixer_get_stats(mixer_ctx* ctx, mixer_stats* st);
 int mixer_set_timing(mixer_ctx* ctx, int enable);
 int mixer_get_timing(mixer_ctx* ctx, int stage, double* total_ms, uint64_t* launches);
 void* mixer_compute_stream(mixer_ctx* ctx);
+
+/* ---- batched correlation of many independent arrays --------------------------------------
+ * (no reference counterpart: the reference correlates one fixed array per process). A batch
+ * holds n_jobs arrays of any supported geometry; each run correlates one integration
+ * (cfg.k_integ samples) of every job with one corner-turn launch, ONE persistent correlator
+ * launch whose workgroups pull (job, channel group, super-block, time slot) work items from an
+ * atomic counter (longest first), one integrate + gridding launch and one image launch. The
+ * launch sequence is captured once as a hipGraph and replayed. Tuning:
+ *   pack       small arrays (<= 256 elements): channels packed per correlator workgroup:
+ *              0 = as many as fit (8 waves), 1 = one channel per workgroup
+ *   tb_target  32-sample blocks per work item when a job is split in time (time slots are
+ *              ceil(n_tb / tb_target), at most 8, only for jobs with fewer than 512 items)
+ *   use_graph  1 = replay a captured hipGraph, 0 = plain launches on the batch stream */
+typedef struct mixer_batch mixer_batch;
+typedef struct mixer_batch_tuning {
+    int32_t pack;
+    int32_t tb_target;
+    int32_t use_graph;
+    int32_t reserved;
+} mixer_batch_tuning;
+typedef struct mixer_batch_info {
+    int32_t n_jobs;
+    int32_t n_items;        /* correlator work items */
+    int32_t n_wgs;          /* persistent correlator workgroups */
+    int32_t graph;          /* 1 if runs replay a hipGraph */
+    uint64_t payload_bytes; /* input bytes of one run (all jobs) */
+    uint64_t exec_ops;      /* int8 MFMA operations of one run */
+    uint64_t device_bytes;  /* device memory held by the batch */
+} mixer_batch_info;
+
+void mixer_batch_default_tuning(mixer_batch_tuning* t);
+/* cfgs[j]: geometry of job j (k_integ samples per run; k_update is ignored). tuning may be NULL. */
+int mixer_batch_create(const mixer_array_cfg* cfgs, int n_jobs, int device,
+                       const mixer_batch_tuning* tuning, mixer_batch** out);
+int mixer_batch_destroy(mixer_batch* b);
+/* d_heaps[j]: job j's header-stripped heaps [k_integ/32][n_elem][payload] in device memory;
+ * they must stay valid until the run completes (mixer_batch_sync). */
+int mixer_batch_run(mixer_batch* b, const uint8_t* const* d_heaps);
+int mixer_batch_sync(mixer_batch* b);
+void* mixer_batch_stream(mixer_batch* b);
+int mixer_batch_get_info(mixer_batch* b, mixer_batch_info* info);
+/* Results of the last completed run of job j: V [n_chan][2][n_elem][n_elem] f32, the image
+ * cube, and (device pointer) the job's int32 accumulator with its number of time slots. */
+int mixer_batch_dump(mixer_batch* b, int job, float* host_v);
+int mixer_batch_read_image(mixer_batch* b, int job, float* img, float* vmax, float* vmin);
+int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots);
+/* Autotuner for a batch: times runs of b on its current inputs (the last d_heaps given to
+ * mixer_batch_run, which must have been called once) for each pack x tb_target candidate and
+ * leaves b on the fastest, returned in *best. */
+int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best);
+/* Fills n bytes of device memory with seeded uniform int8 values in [-127, 127]. */
+int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed);
 
 /* ---- HIP plumbing for hosts without a HIP binding of their own -------------------------------
  * (no reference counterpart: the reference calls the CUDA runtime directly). The Python tests

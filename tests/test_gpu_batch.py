@@ -1,0 +1,125 @@
+"""GPU parity of the batched engine (many arrays of different sizes in one launch sequence)
+against the oracle, through the C ABI: V bit-identical to one f32 rounding of the exact int32
+sums, images within 1e-5 of the FP64 grid + DFT oracle's peak, on every graph replay."""
+import sys
+
+import numpy as np
+import pytest
+
+import mixer_amd as M
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(0x4D495835)
+
+# (n_elem, n_chan, n_tb): FOLD sizes 1..8 blocks and BLOCKED sizes 12..32 blocks, uneven channels
+JOBS = [(32, 3, 2), (64, 5, 3), (96, 2, 2), (128, 7, 4), (256, 4, 5), (160, 3, 2), (384, 2, 2),
+        (512, 2, 3), (1024, 1, 2), (256, 1, 1)]
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cfg_for(N, C, n_tb):
+    gx = 16 if N % 16 == 0 and N // 16 <= 16 else 32
+    gy = N // gx
+    ng = max(32, 1 << int(np.ceil(np.log2(2 * max(gx, gy)))))
+    k = 32 * n_tb
+    return M.default_cfg(n_elem=N, grid_x=gx, n_chan=C, k_update=k, k_integ=k, ng=ng,
+                         pkt_pitch=max(16384, 64 + 64 * C))
+
+
+def heaps(N, C, n_tb):
+    return RNG.integers(-127, 128, size=(n_tb, N, 64 * C), dtype=np.int16).astype(np.int8).view(np.uint8)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert M.device_count() >= 1, "no gfx950 device visible to libmixer"
+    M.check(M.lib().mixer_set_device(0))
+    yield
+
+
+def check_job(b, j, cfg, bb, n_tb, images=True):
+    N, C = cfg.n_elem, cfg.n_chan
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    v = b.dump(j)
+    exact = np.stack([ore, oim], axis=1)
+    assert np.array_equal(v, exact.astype(np.float32)), f"job {j} (N={N} C={C}) V differs"
+    if images:
+        img, vmax, vmin = b.read_image(j)
+        oimg, _, _, _, _ = O.image_from_v(ore, oim, cfg.grid_x, cfg.ng)
+        scale = np.maximum(np.abs(oimg).reshape(C, -1).max(axis=1), 1e-30)
+        err = np.abs(img.astype(np.float64) - oimg).reshape(C, -1).max(axis=1) / scale
+        assert err.max() <= 1e-5, f"job {j} image error {err.max():.3e}"
+        assert np.array_equal(vmax, img.reshape(C, -1).max(1))
+        assert np.array_equal(vmin, img.reshape(C, -1).min(1))
+
+
+@pytest.mark.parametrize("pack,use_graph", [(0, 1), (1, 1), (0, 0)])
+def test_batch_uneven_jobs_bit_exact(pack, use_graph):
+    log(f"batch pack={pack} graph={use_graph} jobs={JOBS}")
+    cfgs = [cfg_for(*j) for j in JOBS]
+    t = M.batch_default_tuning()
+    t.pack, t.use_graph = pack, use_graph
+    with M.Batch(cfgs, tuning=t) as b:
+        info = b.info()
+        assert info.n_jobs == len(JOBS) and info.graph == use_graph and info.n_items > 0
+        bbs = [heaps(*j) for j in JOBS]
+        bufs = [M.DeviceBuffer.from_array(x) for x in bbs]
+        for rep in range(3):
+            log(f"  replay {rep}")
+            b.run(bufs)
+            b.sync()
+            for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, JOBS)):
+                check_job(b, j, cfg, bb, jb[2], images=(rep != 1))
+            # new data in the same buffers: the captured graph must see it
+            bbs = [heaps(*j) for j in JOBS]
+            for x, d in zip(bbs, bufs):
+                d.upload(x)
+        # new buffers: the job table is re-pointed between replays
+        bufs2 = [M.DeviceBuffer.from_array(x) for x in bbs]
+        b.run(bufs2)
+        b.sync()
+        for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, JOBS)):
+            check_job(b, j, cfg, bb, jb[2])
+
+
+def test_batch_time_split_and_autotune():
+    """Long integrations are split over time slots (partial int32 sums); the autotuner's
+    choice must give the same V."""
+    jobs = [(64, 2, 40), (256, 2, 24), (512, 1, 9), (32, 1, 33)]
+    cfgs = [cfg_for(*j) for j in jobs]
+    t = M.batch_default_tuning()
+    t.tb_target = 8
+    with M.Batch(cfgs, tuning=t) as b:
+        bbs = [heaps(*j) for j in jobs]
+        bufs = [M.DeviceBuffer.from_array(x) for x in bbs]
+        b.run(bufs)
+        b.sync()
+        for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):
+            check_job(b, j, cfg, bb, jb[2])
+        best = b.autotune(reps=2)
+        log(f"  autotune -> {best.as_dict()}")
+        assert best.pack in (0, 1) and best.tb_target in (64, 128, 256, 512)
+        b.run(bufs)
+        b.sync()
+        for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):
+            check_job(b, j, cfg, bb, jb[2], images=False)
+
+
+def test_batch_device_fill_matches_host_copy():
+    """mixer_fill_random output, read back and fed to the oracle (the benchmark's input path)."""
+    cfg = cfg_for(128, 3, 4)
+    d = M.DeviceBuffer(4 * 128 * 64 * 3)
+    M.fill_random(d, seed=0x4D495834)
+    M.check(M.lib().mixer_device_sync())
+    bb = d.download(np.uint8, (4, 128, 64 * 3))
+    s = bb.view(np.int8)
+    assert s.min() >= -127 and s.max() <= 127 and len(np.unique(s)) > 200
+    with M.Batch([cfg]) as b:
+        b.run([d])
+        b.sync()
+        check_job(b, 0, cfg, bb, 4)

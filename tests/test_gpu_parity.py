@@ -298,3 +298,51 @@ def test_full_size_cfg2_channels_subset():
     assert (np.diagonal(v[:, 0], axis1=1, axis2=2) > 0).all()
     # star at the centre of every channel's image
     assert all(np.unravel_index(np.argmax(img[c]), (32, 32)) == (16, 16) for c in range(C))
+
+
+# ---------------------------------------------------------------- tuning variants
+@pytest.mark.parametrize("N,C", [(256, 3), (128, 2), (1024, 1), (512, 2)])
+def test_every_launch_variant_bit_exact(N, C):
+    """Every (slots, prefetch, corner-turn width) variant is its own kernel instance: all must
+    give the same bytes and the same exact V."""
+    n_tb = 8
+    bb = rand_heaps(n_tb, N, C)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    are, aim, _, _ = O.unpack(bb, n_tb, N, C, 32)
+    exp_t = np.stack([are.reshape(C, N, n_tb, 32).transpose(0, 2, 1, 3),
+                      aim.reshape(C, N, n_tb, 32).transpose(0, 2, 1, 3)], axis=2)
+    d_bb = M.DeviceBuffer.from_array(bb)
+    try:
+        for slots in (1, 2, 8):
+            for pf in (1, 2, 4):
+                for ct in (8, 16, 32):
+                    if ct != 32 and (slots, pf) != (2, 2):
+                        continue
+                    M.set_tuning(N, slots, pf, ct)
+                    cfg = cfg_for(N, C, 32 * n_tb)
+                    assert M.acc_slots(cfg) == slots
+                    d_t = M.DeviceBuffer(M.tiles_bytes(cfg, n_tb))
+                    d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+                    M.corner_turn(cfg, d_bb, n_tb, d_t)
+                    assert np.array_equal(d_t.download(np.int8, exp_t.shape), exp_t), (slots, pf, ct)
+                    M.xcorr(cfg, d_t, n_tb, d_acc)
+                    vre, vim = M.unpack_acc(d_acc.download(np.int32, (M.acc_bytes(cfg) // 4,)), cfg)
+                    assert np.array_equal(vre, ore) and np.array_equal(vim, oim), (slots, pf, ct)
+    finally:
+        M.lib().mixer_set_tuning(N, M.C.byref(M.Tuning(0, 0, 0, 0)))
+
+
+def test_autotune_sets_a_valid_plan():
+    cfg = cfg_for(128, 16, 1024)
+    try:
+        best = M.autotune(cfg)
+        assert best.slots in (1, 2, 4, 8) and best.prefetch in (1, 2, 4) and best.ct_chan in (8, 16, 32)
+        t = M.get_tuning(cfg)
+        assert (t.slots, t.prefetch, t.ct_chan) == (best.slots, best.prefetch, best.ct_chan)
+        bb = rand_heaps(32, 128, 16)
+        c2, _, acc = gpu_v(cfg, bb, 32)
+        vre, vim = M.unpack_acc(acc, c2)
+        ore, oim = O.v_full_from_heaps(bb, 32, 128, 16, 32)
+        assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
+    finally:
+        M.lib().mixer_set_tuning(128, M.C.byref(M.Tuning(0, 0, 0, 0)))
