@@ -88,21 +88,29 @@ def algorithmic(cfg):
     }
 
 
-def cpu_baseline(cfg, heaps, n_tb, budget_s=15.0):
+def cpu_baseline(cfg, heaps, n_tb, budget_s=12.0):
+    """The oracle's CPU correlator (full-square reference math, int32) on this host: whole
+    integrations of all channels, repeated until about budget_s seconds of CPU work."""
     import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
     C = cfg.n_chan
     t0 = time.perf_counter()
     O.xengine(heaps, n_tb, cfg.n_elem, C, 32, 0, 2, nthreads=threads)
-    dt = time.perf_counter() - t0
-    nch = int(max(2, min(C, budget_s / max(dt / 2, 1e-6))))
+    per_ch = (time.perf_counter() - t0) / 2
+    nch = int(max(2, min(C, budget_s / max(per_ch, 1e-6))))
+    reps, dt = 0, 0.0
     t0 = time.perf_counter()
-    O.xengine(heaps, n_tb, cfg.n_elem, C, 32, 0, nch, nthreads=threads)
-    dt = time.perf_counter() - t0
-    gbs = cfg.n_elem * nch * n_tb * 32 * 2 / dt / 1e9
+    while True:
+        O.xengine(heaps, n_tb, cfg.n_elem, C, 32, 0, nch, nthreads=threads)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or nch < C:
+            break
+    gbs = cfg.n_elem * nch * n_tb * 32 * 2 * reps / dt / 1e9
+    what = f"{reps} x all {C} channels" if nch == C else f"{nch} of {C} channels"
     return {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{nch} of {C} channels of one {n_tb * 32}-sample integration "
-                      f"({cfg.n_elem} elements), full-square reference math, {dt:.1f} s"}
+            "sample": f"{what} of one {n_tb * 32}-sample integration ({cfg.n_elem} elements), "
+                      f"full-square reference math, {dt:.1f} s"}
 
 
 def main():
@@ -169,10 +177,15 @@ def main():
     kern = {}
     for s, name in enumerate(STAGES):
         tot, n = eng.timing(s)
-        kern[name] = {"avg_ms": tot / max(n, 1), "launches": n}
+        if n:
+            kern[name] = {"avg_ms": tot / n, "launches": n}
     eng.set_timing(False)
-    kern["corner_turn"].update(bound="hbm", achieved=alg["corner_turn_bytes"] / (kern["corner_turn"]["avg_ms"] * 1e-3) / 1e9,
-                               unit="GB/s", algorithmic_per_launch=alg["corner_turn_bytes"])
+    fused = "corner_turn" not in kern
+    if fused:
+        kern["xcorr"]["fused_corner_turn"] = True  # stages straight from the heaps
+    else:
+        kern["corner_turn"].update(bound="hbm", achieved=alg["corner_turn_bytes"] / (kern["corner_turn"]["avg_ms"] * 1e-3) / 1e9,
+                                   unit="GB/s", algorithmic_per_launch=alg["corner_turn_bytes"])
     xt = kern["xcorr"]["avg_ms"] * 1e-3
     kern["xcorr"].update(bound="hbm", achieved=alg["payload_bytes"] / xt / 1e9, unit="GB/s",
                          algorithmic_per_launch=alg["payload_bytes"],
@@ -185,10 +198,10 @@ def main():
         for f in ("avg_ms", "achieved", "tops_algorithmic", "tops_executed"):
             if f in k:
                 k[f] = round(k[f], 4)
-    dom = max(STAGES, key=lambda s: kern[s]["avg_ms"])
+    dom = max(kern, key=lambda s: kern[s]["avg_ms"])
     mfma_pct = 100.0 * alg["xcorr_exec_ops"] / xt / 1e12 / INT8_PEAK_TOPS
-    log(f"[bench] kernels: " + ", ".join(f"{s} {kern[s]['avg_ms']:.4f} ms" for s in STAGES) +
-        f"; MFMA {mfma_pct:.1f}% of peak")
+    log(f"[bench] kernels: " + ", ".join(f"{s} {kern[s]['avg_ms']:.4f} ms" for s in kern) +
+        f"; MFMA {mfma_pct:.1f}% of peak" + (" (corner-turn fused into the correlator)" if fused else ""))
 
     # ---- latency: integrations arriving at 10x the 400 GbE-equivalent rate ----
     period = alg["payload_bytes"] / (10 * GBE400_PAYLOAD_GBS * 1e9)

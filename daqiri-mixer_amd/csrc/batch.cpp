@@ -63,10 +63,14 @@ void destroy_graph(mixer_batch* b) {
     b->graph = nullptr;
 }
 
+bool fused(const mixer_batch* b) { return b->tun.staging != 2; }
+
 int enqueue(mixer_batch* b) {
     MIXER_HIP(launch_batch_reset(b->s, b->d_counter));
-    MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
-    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs));
+    if (!fused(b))
+        MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
+    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs,
+                                 fused(b) ? 1 : 0));
     MIXER_HIP(launch_batch_integrate(b->s, b->d_jobs, (int)b->jobs.size(), b->int_total, b->max_ng));
     MIXER_HIP(launch_batch_image(b->s, b->d_jobs, (int)b->jobs.size(), b->img_total, b->max_ng));
     return MIXER_OK;
@@ -171,7 +175,7 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
         // accumulator capacity: the most time slots any tuning candidate (tb_target >= 64) uses
         const int cap = std::min(slots_for(c, 0, 64), 8);
         b->slot_cap.push_back(cap);
-        MIXER_TRY(balloc(b, (void**)&J.tiles, mixer_tiles_bytes(&c, n_tb)));
+        if (!fused(b)) MIXER_TRY(balloc(b, (void**)&J.tiles, mixer_tiles_bytes(&c, n_tb)));
         MIXER_TRY(balloc(b, (void**)&J.acc, (size_t)cap * c.n_chan * NT * 2048 * sizeof(int32_t)));
         MIXER_TRY(balloc(b, (void**)&J.v, (size_t)c.n_chan * 2 * c.n_elem * c.n_elem * sizeof(float)));
         const size_t gbytes = (size_t)c.n_chan * c.ng * c.ng * 16;
@@ -207,7 +211,7 @@ void mixer_batch_default_tuning(mixer_batch_tuning* t) {
     t->pack = 0;
     t->tb_target = 128;
     t->use_graph = 1;
-    t->reserved = 0;
+    t->staging = 0;
 }
 
 int mixer_batch_create(const mixer_array_cfg* cfgs, int n_jobs, int device,

@@ -1,9 +1,9 @@
-// corr_fold_pf1.hip -- FOLD correlator kernels with 1 prefetch stage.
+// corr_fold_pf1.hip -- FOLD correlator kernels with pipeline variant 1.
 #include "corr_impl.h"
 
 namespace mixer {
-hipError_t launch_fold_1(hipStream_t s, const XcorrPlan& p, const int8_t* tiles, int32_t* acc,
-                          int C, int n_tb, int accumulate) {
-    return launch_fold_pf<1>(s, p, tiles, acc, C, n_tb, accumulate);
+hipError_t launch_fold_1(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                          uint32_t* wrap, int C, int n_tb, int accumulate) {
+    return launch_fold_pf<1>(s, p, src, acc, wrap, C, n_tb, accumulate);
 }
 }  // namespace mixer

@@ -1,9 +1,9 @@
-// corr_fold_pf4.hip -- FOLD correlator kernels with 4 prefetch stages.
+// corr_fold_pf4.hip -- FOLD correlator kernels with pipeline variant 4.
 #include "corr_impl.h"
 
 namespace mixer {
-hipError_t launch_fold_4(hipStream_t s, const XcorrPlan& p, const int8_t* tiles, int32_t* acc,
-                          int C, int n_tb, int accumulate) {
-    return launch_fold_pf<4>(s, p, tiles, acc, C, n_tb, accumulate);
+hipError_t launch_fold_4(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                          uint32_t* wrap, int C, int n_tb, int accumulate) {
+    return launch_fold_pf<4>(s, p, src, acc, wrap, C, n_tb, accumulate);
 }
 }  // namespace mixer

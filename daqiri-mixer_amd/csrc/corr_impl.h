@@ -49,15 +49,16 @@ __device__ __forceinline__ void block_mfma(const uint8_t* L, int lrow, int kg, v
 
 // ------------------------------------------------------------------------------------------
 // One wave's share of the correlator over the whole time range (n_kb 32-sample blocks from
-// src, one global slab of `slab` bytes per block). The staged LDS slab holds NBLK element
+// src, addressed through m). The staged LDS slab holds NBLK element
 // blocks (NE = 32*NBLK elements) per time block; GB maps a staged block slot to its element
 // block in the global slab; NTH threads with local ids ltid share the staging. Every wave of
 // the group runs this loop and meets the others at exactly one barrier per stage (plus one).
 // ------------------------------------------------------------------------------------------
 template <int V, int NE, int NBLK, int NTH, int RA, int A0, int A1, int RB, int B0, int B1, typename GB>
-__device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t slab,
+__device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const XSrc& m,
                                           uint8_t* lds0, uint8_t* lds1, int n_kb, GB gblk,
-                                          v16i* ar, v16i* ai, v16i* br, v16i* bi, int ltid) {
+                                          v16i* ar, v16i* ai, v16i* br, v16i* bi, int ltid,
+                                          bool chk, unsigned& saw) {
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
     if constexpr (V == 1) {
         auto ident = [](int b) { return b; };
@@ -69,18 +70,31 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t
                 const int u = ltid + i * NTH;
                 if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
                     const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                    st.r[i] = *(const v4i*)(sl + (size_t)p * (slab >> 1) + gblk(b) * 1024 + eg * 16);
+                    st.r[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
                 }
             }
         };
+        // imaginary -128 (its negation wraps): checked on the staged registers
+        auto check = [&]() {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if ((St::CHUNKS % NTH == 0 || u < St::CHUNKS) && ((u >> 6) & 1))
+                    saw |= has80(st.r[i].x) | has80(st.r[i].y) | has80(st.r[i].z) | has80(st.r[i].w);
+            }
+        };
         gload(src);
+        if (chk) check();
         st.store(lds0, ident, ltid);
         __syncthreads();
         for (int kb = 0; kb < n_kb; ++kb) {
             const bool more = kb + 1 < n_kb;
-            if (more) gload(src + (size_t)(kb + 1) * slab);
+            if (more) gload(src + (size_t)(kb + 1) * m.tb);
             block_mfma<NE, RA, A0, A1, RB, B0, B1>((kb & 1) ? lds1 : lds0, lrow, kg, ar, ai, br, bi);
-            if (more) st.store((kb & 1) ? lds0 : lds1, ident, ltid);
+            if (more) {
+                if (chk) check();
+                st.store((kb & 1) ? lds0 : lds1, ident, ltid);
+            }
             __syncthreads();
         }
     } else {
@@ -88,15 +102,14 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t
         constexpr int NW = NTH / 64, NWIN = NBLK * 2;  // 1 KiB windows per time block
         const int w = ltid >> 6, el = lane >> 1;
         // lane l fills unit l of a window: element el, k-half (l & 1) ^ swizzle(el)
-        const int goff = el * 32 + ((((lane & 1) ^ ((el >> 2) ^ (el >> 3))) & 1) << 4);
+        const int hsw = ((lane & 1) ^ ((el >> 2) ^ (el >> 3))) & 1;
         const int n_st = (n_kb + KB - 1) / KB;
         auto dma = [&](uint8_t* L, int stg) {
             const int kb0 = stg * KB;
             const int kbs = n_kb - kb0 < KB ? n_kb - kb0 : KB;
             for (int win = w; win < kbs * NWIN; win += NW) {
                 const int kbl = win / NWIN, r = win % NWIN, p = r / NBLK, b = r % NBLK;
-                const int8_t* g = src + (size_t)(kb0 + kbl) * slab + (size_t)p * (slab >> 1) +
-                                  gblk(b) * 1024 + goff;
+                const int8_t* g = src + (size_t)(kb0 + kbl) * m.tb + m.off(p, gblk(b) * 32 + el, hsw);
                 uint8_t* l = L + kbl * (NE * 64) + p * (NE * 32) + b * 1024;
                 __builtin_amdgcn_global_load_lds((const void*)g, MIXER_LDS_PTR(l), 16, 0, 0);
             }
@@ -105,8 +118,16 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t
             const int kbs = n_kb - stg * KB < KB ? n_kb - stg * KB : KB;
             static_for<0, KB>([&](auto kc) {
                 constexpr int kbl = decltype(kc)::value;
-                if (kbl < kbs)
+                if (kbl < kbs) {
                     block_mfma<NE, RA, A0, A1, RB, B0, B1>(L + kbl * (NE * 64), lrow, kg, ar, ai, br, bi);
+                    if (chk) {  // imaginary -128: scan the staged imaginary plane (NE * 32 bytes)
+                        const uint8_t* im = L + kbl * (NE * 64) + NE * 32;
+                        for (int u = ltid; u < NE * 2; u += NTH) {
+                            const v4i v = *(const v4i*)(im + u * 16);
+                            saw |= has80(v.x) | has80(v.y) | has80(v.z) | has80(v.w);
+                        }
+                    }
+                }
             });
         };
         dma(lds0, 0);
@@ -126,9 +147,10 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, size_t
 // FOLD wave program: tiles (RA, [A0,A1)) u (RB, [B0,B1)) of the lower triangle of NB blocks;
 // acc points at the channel's tile-packed accumulator (tile t = I(I+1)/2 + J, 2048 ints).
 template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
-__device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, int32_t* __restrict__ acc,
-                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                          int ltid) {
+__device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const XSrc& m,
+                                          int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
+                                          int n_kb, int accumulate, int ltid, bool chk,
+                                          unsigned& saw) {
     constexpr int NA = A1 - A0;
     constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
@@ -137,9 +159,9 @@ __device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, int32_
     for (int j = 0; j < NA; ++j) { ar[j] = (v16i){0}; ai[j] = (v16i){0}; }
 #pragma unroll
     for (int j = 0; j < NBT; ++j) { br[j] = (v16i){0}; bi[j] = (v16i){0}; }
-    corr_wave<V, NB * 32, NB, NTH, RA, A0, A1, RB, B0, B1>(src, (size_t)NB * 32 * 64, lds0, lds1,
-                                                           n_kb, [](int b) { return b; }, ar, ai,
-                                                           br, bi, ltid);
+    corr_wave<V, NB * 32, NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, lds0, lds1, n_kb,
+                                                           [](int b) { return b; }, ar, ai, br,
+                                                           bi, ltid, chk, saw);
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
         int32_t* t = acc + (size_t)(RA * (RA + 1) / 2 + A0 + j) * 2048;
@@ -171,29 +193,30 @@ struct FoldLayout {
 };
 
 template <int V, int NB, int W, int SPLIT_FROM = 6>
-__device__ __forceinline__ void fold_run(const int8_t* src, int32_t* acc, uint8_t* lds0,
-                                         uint8_t* lds1, int n_kb, int accumulate, int ltid) {
+__device__ __forceinline__ void fold_run(const int8_t* src, const XSrc& m, int32_t* acc,
+                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                         int ltid, bool chk, unsigned& saw) {
     using Lay = FoldLayout<NB, SPLIT_FROM>;
     constexpr int NTH = Lay::NTH;
     if constexpr (!Lay::SPLIT) {
         constexpr int R1 = W, R2 = NB - 1 - W;
         if constexpr (R1 == R2)
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
         else
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
     } else {
         constexpr int p = W % Lay::P;
         constexpr int R1 = p, R2 = NB - 1 - p;
         constexpr int T = (R1 == R2) ? R1 + 1 : R1 + R2 + 2;
         constexpr int S = (T + 1) / 2;
         if constexpr (W < Lay::P) {
-            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
         } else if constexpr (R1 == R2) {
-            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
         } else if constexpr (S > R2) {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
         } else {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
         }
     }
 }
@@ -207,31 +230,54 @@ struct FoldLds {
 // One channel (or time slot of it) of the FOLD correlator on the FoldLayout<NB>::NTH threads
 // with local ids ltid in [0, NTH) (the group's wave w = ltid / 64 runs wave program w).
 template <int NB, int V, int SPLIT_FROM = 6>
-__device__ __forceinline__ void fold_item(const int8_t* src, int32_t* acc, uint8_t* lds0,
-                                          uint8_t* lds1, int n_kb, int accumulate, int ltid) {
+__device__ __forceinline__ void fold_item(const int8_t* src, const XSrc& m, int32_t* acc,
+                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                          int ltid, bool chk, unsigned& saw) {
     const int w = ltid >> 6;
     static_for<0, FoldLayout<NB, SPLIT_FROM>::WAVES>([&](auto Wc) {
         constexpr int W = decltype(Wc)::value;
-        if (w == W) fold_run<V, NB, W, SPLIT_FROM>(src, acc, lds0, lds1, n_kb, accumulate, ltid);
+        if (w == W)
+            fold_run<V, NB, W, SPLIT_FROM>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
     });
 }
 
-// FOLD correlator (arrays of NB <= 8 blocks of 32 elements). grid: x = channel, y = time slot.
+// One increment of the -128 counter per workgroup that staged an imaginary -128.
+__device__ __forceinline__ void count_wrap(uint32_t* wrap, unsigned saw) {
+    if (!wrap) return;
+    const int any = __syncthreads_or(saw ? 1 : 0);
+    if (threadIdx.x == 0 && any) atomicAdd(wrap, 1u);
+}
+
+// Workgroup b of a grid padded to 8 * ceil(total / 8): dispatch is round-robin over the 8 XCDs,
+// so b -> (b % 8) * per_xcd + b / 8 gives each XCD a contiguous range of logical work items
+// (a bijection onto [0, 8 * per_xcd); items >= total are idle).
+__device__ __forceinline__ int xcd_logical(int b, int total) {
+    const int per_xcd = (total + 7) / 8;
+    return (b % 8) * per_xcd + b / 8;
+}
+
+// FOLD correlator (arrays of NB <= 8 blocks of 32 elements): one workgroup per (time slot,
+// channel), consecutive channels of a slot on one XCD (4 channels share each 128-byte line of
+// a heap plane, so their workgroups share the L2 when staging straight from the heaps).
 // The whole channel slab (NB blocks) is staged per stage into one of two LDS buffers.
 template <int NB, int V>
 __global__ __launch_bounds__(FoldLayout<NB>::NTH)
-void k_xcorr_fold(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc, int C, int n_tb,
-                  int n_slots, int accumulate) {
+void k_xcorr_fold(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap, int C,
+                  int n_tb, int n_slots, int accumulate) {
     constexpr int NT = NB * (NB + 1) / 2;
     __shared__ __attribute__((aligned(16))) uint8_t lds0[FoldLds<NB, V>::BYTES];
     __shared__ __attribute__((aligned(16))) uint8_t lds1[FoldLds<NB, V>::BYTES];
-    const int c = blockIdx.x, slot = blockIdx.y;
+    const int logical = xcd_logical(blockIdx.x, C * n_slots);
+    if (logical >= C * n_slots) return;  // uniform over the workgroup
+    const int c = logical % C, slot = logical / C;
     const int kb0 = (int)((long)n_tb * slot / n_slots);
     const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
-    if (kb1 <= kb0) return;  // uniform over the workgroup
+    if (kb1 <= kb0) return;
     int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
-    const int8_t* src = tiles + ((size_t)c * n_tb + kb0) * (size_t)NB * 32 * 64;
-    fold_item<NB, V>(src, a, lds0, lds1, kb1 - kb0, accumulate, threadIdx.x);
+    const int8_t* src = xs.base + (size_t)c * xs.chan + (size_t)kb0 * xs.tb;
+    unsigned saw = 0;
+    fold_item<NB, V>(src, xs, a, lds0, lds1, kb1 - kb0, accumulate, threadIdx.x, wrap != nullptr, saw);
+    count_wrap(wrap, saw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -246,21 +292,21 @@ void k_xcorr_fold(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc, i
 // per XCD L2.
 // ------------------------------------------------------------------------------------------
 template <int V, int PAT, int RS, int CS, typename GB>
-__device__ __forceinline__ void blk_prog(const int8_t* __restrict__ src, size_t slab,
+__device__ __forceinline__ void blk_prog(const int8_t* __restrict__ src, const XSrc& m,
                                          int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
                                          int n_kb, GB gblk, int Ib, int Jb, int accumulate,
-                                         int ltid) {
+                                         int ltid, bool chk, unsigned& saw) {
     // PAT 0 FULL: rows (RS, RS+1) x columns (CS, CS+1); 1 DIAG: (0,0), (1,0), (1,1); 2 EMPTY
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
     v16i ar[2], ai[2], br[2], bi[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) { ar[j] = (v16i){0}; ai[j] = (v16i){0}; br[j] = (v16i){0}; bi[j] = (v16i){0}; }
     if constexpr (PAT == 0)
-        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 2, RS + 1, CS, CS + 2>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 2, RS + 1, CS, CS + 2>(src, m, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid, chk, saw);
     else if constexpr (PAT == 1)
-        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 1, RS + 1, CS, CS + 2>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 1, RS + 1, CS, CS + 2>(src, m, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid, chk, saw);
     else
-        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS, -1, 0, 0>(src, slab, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid);
+        corr_wave<V, 12 * 32, 12, 512, RS, CS, CS, -1, 0, 0>(src, m, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid, chk, saw);
     if constexpr (PAT != 2) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -282,11 +328,9 @@ struct BlkLds {
 
 // One super-block (I0, J0) of the BLOCKED correlator on 512 threads with local ids ltid.
 template <int NB, int V>
-__device__ __forceinline__ void blk_item(const int8_t* src, int32_t* a, uint8_t* lds0,
-                                         uint8_t* lds1, int n_kb, int accumulate, int I0, int J0,
-                                         int ltid) {
-    constexpr int NE = NB * 32;
-    const size_t slab = (size_t)NE * 64;
+__device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32_t* a,
+                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                         int I0, int J0, int ltid, bool chk, unsigned& saw) {
     auto gblk = [=](int s) {
         const int g = s < 4 ? I0 + s : J0 + (s - 4);
         return g < NB ? g : NB - 1;  // columns past the array only feed EMPTY waves
@@ -298,28 +342,25 @@ __device__ __forceinline__ void blk_item(const int8_t* src, int32_t* a, uint8_t*
         if (w == W) {
             const int Ib = I0 + RS, Jb = J0 + CS - 4;
             if (Jb + 1 < Ib)
-                blk_prog<V, 0, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+                blk_prog<V, 0, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
             else if (Jb == Ib)
-                blk_prog<V, 1, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+                blk_prog<V, 1, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
             else
-                blk_prog<V, 2, RS, CS>(src, slab, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid);
+                blk_prog<V, 2, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
         }
     });
 }
 
 template <int NB, int V>
 __global__ __launch_bounds__(512)
-void k_xcorr_blk(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc,
+void k_xcorr_blk(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap,
                  const int2* __restrict__ sblocks, int n_sb, int C, int n_tb, int n_slots,
                  int accumulate) {
-    constexpr int NE = NB * 32, NT = NB * (NB + 1) / 2;
+    constexpr int NT = NB * (NB + 1) / 2;
     __shared__ __attribute__((aligned(16))) uint8_t lds0[BlkLds<V>::BYTES];
     __shared__ __attribute__((aligned(16))) uint8_t lds1[BlkLds<V>::BYTES];
     const int total = n_sb * C * n_slots;
-    const int b = blockIdx.x;
-    const int per_xcd = (total + 7) / 8;
-    // the grid is padded to 8 * per_xcd, so b -> (b % 8) * per_xcd + b / 8 is a bijection
-    const int logical = (b % 8) * per_xcd + b / 8;
+    const int logical = xcd_logical(blockIdx.x, total);
     if (logical >= total) return;
     const int slot = logical / (n_sb * C);
     const int rem = logical % (n_sb * C);
@@ -328,58 +369,61 @@ void k_xcorr_blk(const int8_t* __restrict__ tiles, int32_t* __restrict__ acc,
     const int kb0 = (int)((long)n_tb * slot / n_slots);
     const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
     if (kb1 <= kb0) return;
-    const size_t slab = (size_t)NE * 64;
-    const int8_t* src = tiles + ((size_t)c * n_tb + kb0) * slab;
+    const int8_t* src = xs.base + (size_t)c * xs.chan + (size_t)kb0 * xs.tb;
     int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
-    blk_item<NB, V>(src, a, lds0, lds1, kb1 - kb0, accumulate, sb.x, sb.y, threadIdx.x);
+    unsigned saw = 0;
+    blk_item<NB, V>(src, xs, a, lds0, lds1, kb1 - kb0, accumulate, sb.x, sb.y, threadIdx.x,
+                    wrap != nullptr, saw);
+    count_wrap(wrap, saw);
 }
 
 // ------------------------------------------------------------------------------------------
 // Launch helpers, instantiated per variant in corr_{fold,blk}_pf{1,2,4}.hip
 // ------------------------------------------------------------------------------------------
 template <int NB, int V>
-inline hipError_t launch_fold_t(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
-                                int32_t* acc, int C, int n_tb, int accumulate) {
-    hipLaunchKernelGGL((k_xcorr_fold<NB, V>), dim3(C, p.n_slots), dim3(FoldLayout<NB>::NTH), 0, s,
-                       tiles, acc, C, n_tb, p.n_slots, accumulate);
+inline hipError_t launch_fold_t(hipStream_t s, const XcorrPlan& p, const XSrc& src,
+                                int32_t* acc, uint32_t* wrap, int C, int n_tb, int accumulate) {
+    const int wgs = C * p.n_slots;
+    hipLaunchKernelGGL((k_xcorr_fold<NB, V>), dim3((wgs + 7) / 8 * 8), dim3(FoldLayout<NB>::NTH),
+                       0, s, src, acc, wrap, C, n_tb, p.n_slots, accumulate);
     return hipGetLastError();
 }
 
 template <int NB, int V>
-inline hipError_t launch_blk_t(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
-                               int32_t* acc, int C, int n_tb, int accumulate) {
+inline hipError_t launch_blk_t(hipStream_t s, const XcorrPlan& p, const XSrc& src,
+                               int32_t* acc, uint32_t* wrap, int C, int n_tb, int accumulate) {
     const int wgs = p.n_tasks * C * p.n_slots;
-    hipLaunchKernelGGL((k_xcorr_blk<NB, V>), dim3((wgs + 7) / 8 * 8), dim3(512), 0, s, tiles, acc,
-                       (const int2*)p.d_tasks, p.n_tasks, C, n_tb, p.n_slots, accumulate);
+    hipLaunchKernelGGL((k_xcorr_blk<NB, V>), dim3((wgs + 7) / 8 * 8), dim3(512), 0, s, src, acc,
+                       wrap, (const int2*)p.d_tasks, p.n_tasks, C, n_tb, p.n_slots, accumulate);
     return hipGetLastError();
 }
 
 template <int V>
-inline hipError_t launch_fold_pf(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
-                                 int32_t* acc, int C, int n_tb, int accumulate) {
+inline hipError_t launch_fold_pf(hipStream_t s, const XcorrPlan& p, const XSrc& src,
+                                 int32_t* acc, uint32_t* wrap, int C, int n_tb, int accumulate) {
     switch (p.n_elem / 32) {
-    case 1: return launch_fold_t<1, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 2: return launch_fold_t<2, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 3: return launch_fold_t<3, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 4: return launch_fold_t<4, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 5: return launch_fold_t<5, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 6: return launch_fold_t<6, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 7: return launch_fold_t<7, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 8: return launch_fold_t<8, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 1: return launch_fold_t<1, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 2: return launch_fold_t<2, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 3: return launch_fold_t<3, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 4: return launch_fold_t<4, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 5: return launch_fold_t<5, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 6: return launch_fold_t<6, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 7: return launch_fold_t<7, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 8: return launch_fold_t<8, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <int V>
-inline hipError_t launch_blk_pf(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,
-                                int32_t* acc, int C, int n_tb, int accumulate) {
+inline hipError_t launch_blk_pf(hipStream_t s, const XcorrPlan& p, const XSrc& src,
+                                int32_t* acc, uint32_t* wrap, int C, int n_tb, int accumulate) {
     switch (p.n_elem / 32) {
-    case 12: return launch_blk_t<12, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 16: return launch_blk_t<16, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 20: return launch_blk_t<20, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 24: return launch_blk_t<24, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 28: return launch_blk_t<28, V>(s, p, tiles, acc, C, n_tb, accumulate);
-    case 32: return launch_blk_t<32, V>(s, p, tiles, acc, C, n_tb, accumulate);
+    case 12: return launch_blk_t<12, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 16: return launch_blk_t<16, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 20: return launch_blk_t<20, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 24: return launch_blk_t<24, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 28: return launch_blk_t<28, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 32: return launch_blk_t<32, V>(s, p, src, acc, wrap, C, n_tb, accumulate);
     default: return hipErrorInvalidValue;
     }
 }

@@ -189,8 +189,10 @@ int finish_integration(mixer_ctx* x) {
     return MIXER_OK;
 }
 
-// Corner-turn + correlate one V-update whose heaps start at src (device).
+// Corner-turn of one V-update whose heaps start at src (device); nothing when the correlator
+// stages straight from the heaps.
 int run_update(mixer_ctx* x, const uint8_t* src) {
+    if (x->plan.fused) return MIXER_OK;
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0));
     MIXER_HIP(launch_corner_turn(x->s_comp, src, x->d_tiles, x->d_wrap, x->cfg.n_elem,
@@ -198,11 +200,14 @@ int run_update(mixer_ctx* x, const uint8_t* src) {
     MIXER_TRY(time_end(x, 0, t0));
     return MIXER_OK;
 }
-int run_gemm(mixer_ctx* x) {
+// Correlate one V-update (from the heaps at src when fused, else from the tiles).
+int run_gemm(mixer_ctx* x, const uint8_t* src) {
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0));
-    MIXER_HIP(launch_xcorr(x->s_comp, x->plan, x->d_tiles, x->d_acc, x->cfg.n_chan, x->n_tb_upd,
-                           x->upd_in_integ > 0 ? 1 : 0));
+    const XSrc xs = x->plan.fused ? xsrc_heaps(src, x->cfg.n_elem, x->cfg.n_chan)
+                                  : xsrc_tiles(x->d_tiles, x->cfg.n_elem, x->n_tb_upd);
+    MIXER_HIP(launch_xcorr(x->s_comp, x->plan, xs, x->d_acc, x->plan.fused ? x->d_wrap : nullptr,
+                           x->cfg.n_chan, x->n_tb_upd, x->upd_in_integ > 0 ? 1 : 0));
     MIXER_TRY(time_end(x, 1, t0));
     x->upd_in_integ++;
     x->st.vupdates++;
@@ -218,10 +223,11 @@ int dispatch_ring(mixer_ctx* x) {
     MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
     const bool flop = !(x->cfg.flags & MIXER_CFG_NOFLOP);
     if (flop) MIXER_TRY(run_update(x, x->d_ring[b]));
+    if (flop && x->plan.fused) MIXER_TRY(run_gemm(x, x->d_ring[b]));  // reads the ring itself
     MIXER_HIP(hipEventRecord(x->ring_free[b], x->s_comp));
     x->ring_used[b] = true;
-    if (flop) MIXER_TRY(run_gemm(x));
-    else x->st.time_blocks += x->n_tb_upd;
+    if (flop && !x->plan.fused) MIXER_TRY(run_gemm(x, x->d_ring[b]));
+    if (!flop) x->st.time_blocks += x->n_tb_upd;
     x->ring_cur ^= 1;
     x->ring_fill = 0;
     if (x->ring_used[x->ring_cur]) MIXER_HIP(hipStreamWaitEvent(x->s_copy, x->ring_free[x->ring_cur], 0));
@@ -305,7 +311,7 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
         MIXER_HIP(hipMalloc(&x->d_vcum, x->v_bytes));
         MIXER_HIP(hipMemset(x->d_vcum, 0, x->v_bytes));  // zeroed once (src/rx_host_corr.cu:188)
     }
-    MIXER_HIP(hipMalloc(&x->d_tiles, mixer_tiles_bytes(cfg, x->n_tb_upd)));
+    if (!x->plan.fused) MIXER_HIP(hipMalloc(&x->d_tiles, mixer_tiles_bytes(cfg, x->n_tb_upd)));
     MIXER_HIP(hipMalloc(&x->d_acc, mixer_acc_bytes(cfg)));
     MIXER_HIP(hipMalloc(&x->d_wrap, sizeof(uint32_t)));
     MIXER_HIP(hipMemset(x->d_wrap, 0, sizeof(uint32_t)));
@@ -411,7 +417,7 @@ int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arriv
     while (n_tb - done >= x->n_tb_upd) {
         if (!(x->cfg.flags & MIXER_CFG_NOFLOP)) {
             MIXER_TRY(run_update(x, d + (size_t)done * x->tb_bytes));
-            MIXER_TRY(run_gemm(x));
+            MIXER_TRY(run_gemm(x, d + (size_t)done * x->tb_bytes));
         }
         done += x->n_tb_upd;
         x->st.heaps_in += (uint64_t)x->n_tb_upd * x->cfg.n_elem;

@@ -16,16 +16,41 @@ struct XcorrPlan {
     enum Kind { FOLD = 0, BLOCKED = 1 } kind;
     int n_elem;
     int n_slots;
-    int prefetch;         // register prefetch stages of the time loop: 1, 2 or 4
+    int prefetch;         // time-loop pipeline: 1 = registers one block ahead, 2/4 = LDS DMA stages
     int ct_ch;            // corner-turn channels per workgroup: 8, 16 or 32
+    int fused;            // contexts: 1 = correlate straight from the heaps, 0 = via tiles
     int n_tasks;          // BLOCKED only
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
 };
 
+// Where the correlator reads its 32-sample element rows. A row (channel c, time block tb,
+// plane p, element e) is 32 contiguous bytes both in the corner-turned tiles
+// [C][n_tb][2][N][32] and in the raw heaps [n_tb][N][re C*32 | im C*32], so the correlator can
+// stage straight from the heaps (the corner-turn becomes part of its loads) or from tiles.
+struct XSrc {
+    const int8_t* base;  // element 0, plane 0, time block 0 of channel 0
+    size_t chan;         // bytes between channels
+    size_t tb;           // bytes between time blocks
+    int plane;           // bytes between the re and im planes
+    int elem;            // bytes between elements
+    __device__ __forceinline__ size_t off(int p, int e, int h) const {
+        return (size_t)p * plane + (size_t)e * elem + h * 16;
+    }
+};
+__host__ __device__ inline XSrc xsrc_tiles(const int8_t* tiles, int N, int n_tb) {
+    const size_t slab = (size_t)N * 64;
+    return XSrc{tiles, (size_t)n_tb * slab, slab, (int)(slab / 2), 32};
+}
+__host__ __device__ inline XSrc xsrc_heaps(const uint8_t* heaps, int N, int C) {
+    const size_t payload = (size_t)C * 64;
+    return XSrc{(const int8_t*)heaps, 32, (size_t)N * payload, C * 32, (int)payload};
+}
+
 hipError_t launch_corner_turn(hipStream_t s, const uint8_t* heaps, int8_t* tiles, uint32_t* wrap,
                               int N, int C, int n_tb, int c_base, int c_count, int cch);
-hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& plan, const int8_t* tiles, int32_t* acc,
-                        int C, int n_tb, int accumulate);
+// wrap (optional): counts workgroups that staged an imaginary -128
+hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& plan, const XSrc& src, int32_t* acc,
+                        uint32_t* wrap, int C, int n_tb, int accumulate);
 hipError_t launch_integrate(hipStream_t s, const int32_t* acc, float* v_store, float* v_add,
                             unsigned long long* G, int N, int C, int n_slots, int grid_x, int ng);
 size_t image_lds_bytes(int ng);
@@ -56,8 +81,9 @@ struct BatchJobDev {
 // Correlator work item: (job, first channel, super-block index or 0, time slot)
 hipError_t launch_batch_reset(hipStream_t s, int* counter);
 hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
+// fused: stage from each job's heaps (no corner-turn launch) instead of its tiles
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs);
+                              int n_items, int* counter, int n_wgs, int fused);
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
                                   int max_ng);
 hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,

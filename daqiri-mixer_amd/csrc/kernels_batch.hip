@@ -40,8 +40,9 @@ __global__ __launch_bounds__(256) void k_batch_corner_turn(const BatchJobDev* __
 // columns [a0, a0+na) and row B with columns [b0, b0+nb) (LDS slots; global *_g), accumulator
 // slot s < na for row A and na <= s < na+nb for row B, so accumulator indices stay static.
 // The group's nth threads (local ids ltid; ltid < 0 for waves outside any group, which only
-// meet the barriers) stage nblk element blocks (global block gblk(b)) per 32-sample block
-// through registers into two LDS buffers (one barrier per block, like corr_wave with V = 1).
+// meet the barriers) stage nblk element blocks (global block gblk(b)) per 32-sample block,
+// addressed through m (tiles or heaps), through registers into two LDS buffers (one barrier
+// per block, like corr_wave with V = 1).
 // ------------------------------------------------------------------------------------------
 constexpr int kMaxT = 5;
 constexpr int kMaxStage = 3;  // 16-byte chunks staged per thread per block (NBLK*128 <= 3*512)
@@ -63,7 +64,7 @@ __device__ __forceinline__ Frag load_frag_rt(const uint8_t* L, int ne, int B, in
 }
 
 template <typename GB>
-__device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, size_t slab, int nblk,
+__device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, const XSrc& m, int nblk,
                                              GB gblk, uint8_t* lds0, uint8_t* lds1, int n_kb,
                                              const WaveTiles& wt, bool active, int32_t* __restrict__ acc,
                                              int ltid, int nth) {
@@ -79,7 +80,7 @@ __device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, siz
             const int u = ltid + i * nth;
             if (ltid >= 0 && u < chunks) {
                 const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                st[i] = *(const v4i*)(sl + (size_t)p * (slab >> 1) + gblk(b) * 1024 + eg * 16);
+                st[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
             }
         }
     };
@@ -98,7 +99,7 @@ __device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, siz
     __syncthreads();
     for (int kb = 0; kb < n_kb; ++kb) {
         const bool more = kb + 1 < n_kb;
-        if (more) gload(src + (size_t)(kb + 1) * slab);
+        if (more) gload(src + (size_t)(kb + 1) * m.tb);
         if (active) {
             const uint8_t* L = (kb & 1) ? lds1 : lds0;
             const Frag fa = load_frag_rt(L, ne, wt.ra, lrow, kg);
@@ -182,7 +183,7 @@ constexpr int kBatchLds = 48 * 1024;
 // so the longest start first and the tail is made of short items.
 __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restrict__ jobs,
                                                      const int4* __restrict__ items, int n_items,
-                                                     int* __restrict__ counter) {
+                                                     int* __restrict__ counter, int fused) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[kBatchLds];
     __shared__ int item_sh;
     const int w8 = threadIdx.x >> 6;
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
         if (kb1 > kb0) {  // uniform
             // One call site for every array size (a second inlined copy makes the allocator
             // spill): fold channels and BLOCKED super-blocks differ only in these parameters.
-            const size_t slab = (size_t)nb * 32 * 64;
+            const XSrc m = fused ? xsrc_heaps(J.heaps, J.n_elem, J.n_chan)
+                                 : xsrc_tiles(J.tiles, J.n_elem, J.n_tb);
             int nblk, buf, ltid, nth, c, row0, col0;
             bool active = false, have = true;
             WaveTiles wt;
@@ -235,9 +237,9 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
                 const int g = b < 4 ? row0 + b : col0 + (b - 4);
                 return g < nb ? g : nb - 1;  // columns past the array only feed idle waves
             };
-            const int8_t* src = J.tiles + ((size_t)c * J.n_tb + kb0) * slab;
+            const int8_t* src = m.base + (size_t)c * m.chan + (size_t)kb0 * m.tb;
             int32_t* a = J.acc + ((size_t)slot * J.n_chan + c) * NT * 2048;
-            corr_wave_rt(src, slab, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
+            corr_wave_rt(src, m, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
                          ltid, nth);
         }
         __syncthreads();  // LDS reuse by the next item
@@ -276,8 +278,9 @@ hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int 
 }
 
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs) {
-    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter);
+                              int n_items, int* counter, int n_wgs, int fused) {
+    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter,
+                       fused);
     return hipGetLastError();
 }
 

@@ -5,8 +5,9 @@
 namespace mixer {
 
 #define MIXER_DECL(kind, pf)                                                                  \
-    hipError_t launch_##kind##_##pf(hipStream_t s, const XcorrPlan& p, const int8_t* tiles,  \
-                                    int32_t* acc, int C, int n_tb, int accumulate);
+    hipError_t launch_##kind##_##pf(hipStream_t s, const XcorrPlan& p, const XSrc& src,      \
+                                    int32_t* acc, uint32_t* wrap, int C, int n_tb,           \
+                                    int accumulate);
 MIXER_DECL(fold, 1)
 MIXER_DECL(fold, 2)
 MIXER_DECL(fold, 4)
@@ -15,16 +16,16 @@ MIXER_DECL(blk, 2)
 MIXER_DECL(blk, 4)
 #undef MIXER_DECL
 
-hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const int8_t* tiles, int32_t* acc,
-                        int C, int n_tb, int accumulate) {
+hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                        uint32_t* wrap, int C, int n_tb, int accumulate) {
     const bool fold = p.kind == XcorrPlan::FOLD;
     switch (p.prefetch) {
-    case 1: return fold ? launch_fold_1(s, p, tiles, acc, C, n_tb, accumulate)
-                        : launch_blk_1(s, p, tiles, acc, C, n_tb, accumulate);
-    case 2: return fold ? launch_fold_2(s, p, tiles, acc, C, n_tb, accumulate)
-                        : launch_blk_2(s, p, tiles, acc, C, n_tb, accumulate);
-    case 4: return fold ? launch_fold_4(s, p, tiles, acc, C, n_tb, accumulate)
-                        : launch_blk_4(s, p, tiles, acc, C, n_tb, accumulate);
+    case 1: return fold ? launch_fold_1(s, p, src, acc, wrap, C, n_tb, accumulate)
+                        : launch_blk_1(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 2: return fold ? launch_fold_2(s, p, src, acc, wrap, C, n_tb, accumulate)
+                        : launch_blk_2(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 4: return fold ? launch_fold_4(s, p, src, acc, wrap, C, n_tb, accumulate)
+                        : launch_blk_4(s, p, src, acc, wrap, C, n_tb, accumulate);
     default: return hipErrorInvalidValue;
     }
 }

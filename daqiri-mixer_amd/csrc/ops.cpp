@@ -90,7 +90,11 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     p->n_elem = c.n_elem;
     p->n_slots = plan_slots(c);
     const mixer_tuning t = tuning_for(c.n_elem);
-    p->prefetch = t.prefetch > 0 ? t.prefetch : 1;
+    // automatic choices, from timings of whole integrations on MI355X: FOLD arrays correlate
+    // fastest straight from the heaps with 4-block DMA stages; BLOCKED arrays reuse each
+    // element block across many super-blocks and run faster from corner-turned tiles
+    p->fused = t.staging == 1 ? 1 : t.staging == 2 ? 0 : (nb <= 8 ? 1 : 0);
+    p->prefetch = t.prefetch > 0 ? t.prefetch : (p->fused && nb <= 8 ? 4 : 1);
     p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
     p->n_tasks = 0;
     p->d_tasks = nullptr;
@@ -253,8 +257,20 @@ int mixer_xcorr(void* stream, const mixer_array_cfg* c, const int8_t* d_tiles, i
     XcorrPlan p;
     if ((st = get_plan(*c, &p))) return st;
     if (n_tb < p.n_slots) return MIXER_ERR_ARG;
-    return hip_status(launch_xcorr((hipStream_t)stream, p, d_tiles, d_acc, c->n_chan, n_tb,
-                                   accumulate ? 1 : 0));
+    return hip_status(launch_xcorr((hipStream_t)stream, p, xsrc_tiles(d_tiles, c->n_elem, n_tb),
+                                   d_acc, nullptr, c->n_chan, n_tb, accumulate ? 1 : 0));
+}
+
+int mixer_xcorr_heaps(void* stream, const mixer_array_cfg* c, const uint8_t* d_heaps, int n_tb,
+                      int32_t* d_acc, int accumulate, uint32_t* d_wrap) {
+    int st = mixer_validate_cfg(c);
+    if (st) return st;
+    if (!d_heaps || !d_acc || n_tb <= 0) return MIXER_ERR_ARG;
+    XcorrPlan p;
+    if ((st = get_plan(*c, &p))) return st;
+    if (n_tb < p.n_slots) return MIXER_ERR_ARG;
+    return hip_status(launch_xcorr((hipStream_t)stream, p, xsrc_heaps(d_heaps, c->n_elem, c->n_chan),
+                                   d_acc, d_wrap, c->n_chan, n_tb, accumulate ? 1 : 0));
 }
 
 int mixer_integrate(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, float* d_v,
@@ -296,6 +312,7 @@ int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
     if (t->slots < 0 || t->slots > 8) return MIXER_ERR_ARG;
     if (t->prefetch != 0 && t->prefetch != 1 && t->prefetch != 2 && t->prefetch != 4) return MIXER_ERR_ARG;
     if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
+    if (t->staging < 0 || t->staging > 2) return MIXER_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     g_tuning[n_elem] = *t;
     return MIXER_OK;
@@ -305,12 +322,12 @@ int mixer_get_tuning(const mixer_array_cfg* c, mixer_tuning* t) {
     int st = mixer_validate_cfg(c);
     if (st) return st;
     if (!t) return MIXER_ERR_ARG;
-    const int nb = c->n_elem / 32;
-    t->slots = plan_slots(*c);
-    const mixer_tuning u = tuning_for(c->n_elem);
-    t->prefetch = u.prefetch > 0 ? u.prefetch : 1;
-    t->ct_chan = u.ct_chan > 0 ? u.ct_chan : 8;
-    t->reserved = nb <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
+    XcorrPlan p;
+    if ((st = get_plan(*c, &p))) return st;
+    t->slots = p.n_slots;
+    t->prefetch = p.prefetch;
+    t->ct_chan = p.ct_ch;
+    t->staging = p.fused ? 1 : 2;
     return MIXER_OK;
 }
 

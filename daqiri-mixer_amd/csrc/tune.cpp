@@ -32,13 +32,17 @@ struct Bufs {
     }
 };
 
-// One pass of corner-turn + correlate + integrate + image (or the corner-turn alone).
+// One pass of (corner-turn +) correlate + integrate + image (or the corner-turn alone), as a
+// context runs it under the tuning in effect.
 int run_once(const mixer_array_cfg& c, Bufs& b, int n_tb, bool ct_only) {
     XcorrPlan p;
     MIXER_TRY(get_plan(c, &p));
-    MIXER_HIP(launch_corner_turn(b.s, b.heaps, b.tiles, nullptr, c.n_elem, c.n_chan, n_tb, 0, c.n_chan, p.ct_ch));
+    if (!p.fused || ct_only)
+        MIXER_HIP(launch_corner_turn(b.s, b.heaps, b.tiles, nullptr, c.n_elem, c.n_chan, n_tb, 0,
+                                     c.n_chan, p.ct_ch));
     if (ct_only) return MIXER_OK;
-    MIXER_HIP(launch_xcorr(b.s, p, b.tiles, b.acc, c.n_chan, n_tb, 0));
+    const XSrc xs = p.fused ? xsrc_heaps(b.heaps, c.n_elem, c.n_chan) : xsrc_tiles(b.tiles, c.n_elem, n_tb);
+    MIXER_HIP(launch_xcorr(b.s, p, xs, b.acc, nullptr, c.n_chan, n_tb, 0));
     const float2* tw = nullptr;
     MIXER_TRY(get_twiddles(c.ng, &tw));
     MIXER_HIP(launch_integrate(b.s, b.acc, b.v, nullptr, b.G, c.n_elem, c.n_chan, p.n_slots,
@@ -92,19 +96,23 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
     mixer_tuning t{0, 0, 32, 0}, win{0, 0, 32, 0};
     float best_ms = 1e30f;
     int st = MIXER_OK;
-    for (int slots : {1, 2, 4, 8}) {
-        if (slots > n_tb) continue;
-        for (int pf : {1, 2, 4}) {
-            t.slots = slots;
-            t.prefetch = pf;
-            if ((st = mixer_set_tuning(c.n_elem, &t))) break;
-            float ms = 0;
-            if ((st = time_it(c, b, n_tb, false, &ms))) break;
-            if (ms < best_ms) { best_ms = ms; win = t; }
+    for (int staging : {1, 2}) {
+        for (int slots : {1, 2, 4, 8}) {
+            if (slots > n_tb) continue;
+            for (int pf : {1, 2, 4}) {
+                t.slots = slots;
+                t.prefetch = pf;
+                t.staging = staging;
+                if ((st = mixer_set_tuning(c.n_elem, &t))) break;
+                float ms = 0;
+                if ((st = time_it(c, b, n_tb, false, &ms))) break;
+                if (ms < best_ms) { best_ms = ms; win = t; }
+            }
+            if (st) break;
         }
         if (st) break;
     }
-    if (!st) {
+    if (!st && win.staging == 2) {  // the corner-turn width matters only with a separate pass
         best_ms = 1e30f;
         for (int ct : {8, 16, 32}) {
             t = win;
@@ -116,7 +124,8 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
         }
     }
     if (st) {  // leave the table as it was
-        if (had) (void)mixer_set_tuning(c.n_elem, &saved);
+        const mixer_tuning automatic{0, 0, 0, 0};
+        (void)mixer_set_tuning(c.n_elem, had ? &saved : &automatic);
         return st;
     }
     MIXER_TRY(mixer_set_tuning(c.n_elem, &win));

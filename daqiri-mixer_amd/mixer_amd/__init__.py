@@ -50,18 +50,20 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("staging", C.c_int32)]
 
     def as_dict(self):
-        return {"slots": self.slots, "prefetch": self.prefetch, "ct_chan": self.ct_chan}
+        return {"slots": self.slots, "prefetch": self.prefetch, "ct_chan": self.ct_chan,
+                "staging": self.staging}
 
 
 class BatchTuning(C.Structure):
     _fields_ = [("pack", C.c_int32), ("tb_target", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("staging", C.c_int32)]
 
     def as_dict(self):
-        return {"pack": self.pack, "tb_target": self.tb_target, "use_graph": self.use_graph}
+        return {"pack": self.pack, "tb_target": self.tb_target, "use_graph": self.use_graph,
+                "staging": self.staging}
 
 
 class BatchInfo(C.Structure):
@@ -94,6 +96,7 @@ SIGNATURES = [
     ("mixer_acc_slots", I32, [P]),
     ("mixer_xcorr", I32, [P, P, P, I32, P, I32]),
     ("mixer_integrate", I32, [P, P, P, P, I32]),
+    ("mixer_xcorr_heaps", I32, [P, P, P, I32, P, I32, P]),
     ("mixer_image", I32, [P, P, P, P, P, P]),
     ("mixer_set_tuning", I32, [I32, P]),
     ("mixer_get_tuning", I32, [P, P]),
@@ -268,6 +271,13 @@ def xcorr(cfg, d_tiles: DeviceBuffer, n_tb: int, d_acc: DeviceBuffer, accumulate
           "mixer_xcorr")
 
 
+def xcorr_heaps(cfg, d_heaps: DeviceBuffer, n_tb: int, d_acc: DeviceBuffer, accumulate=False, d_wrap=None,
+                stream=None):
+    """Fused corner-turn + correlate straight from header-free device heaps."""
+    check(lib().mixer_xcorr_heaps(stream, C.byref(cfg), d_heaps.ptr, n_tb, d_acc.ptr, 1 if accumulate else 0,
+                                  d_wrap.ptr if d_wrap is not None else None), "mixer_xcorr_heaps")
+
+
 def integrate(cfg, d_acc: DeviceBuffer, d_v: DeviceBuffer, accumulate=False, stream=None):
     check(lib().mixer_integrate(stream, C.byref(cfg), d_acc.ptr, d_v.ptr, 1 if accumulate else 0),
           "mixer_integrate")
@@ -282,8 +292,8 @@ def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):
           "mixer_xcorr_plain")
 
 
-def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0):
-    t = Tuning(slots, prefetch, ct_chan, 0)
+def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0, staging=0):
+    t = Tuning(slots, prefetch, ct_chan, staging)
     check(lib().mixer_set_tuning(n_elem, C.byref(t)), "mixer_set_tuning")
 
 

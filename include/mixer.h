@@ -120,6 +120,14 @@ int mixer_xcorr(void* stream, const mixer_array_cfg* cfg, const int8_t* d_tiles,
 int mixer_integrate(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_v,
                     int accumulate);
 
+/* Fused corner-turn + correlate: k_packAB + cgemm_blk_t<ACC> (src/rx_host_corr.cu:61-72,
+ * src/cgemm.cuh:84-125) in one launch. The correlator stages each 32-sample element row
+ * straight from the header-free heaps d_heaps [n_tb][n_elem][re C*32 | im C*32] (every row is
+ * one 32-byte run of a heap plane), so no tiles are written or read. Same accumulator as
+ * mixer_xcorr. d_wrap (optional) counts workgroups that staged an imaginary -128. */
+int mixer_xcorr_heaps(void* stream, const mixer_array_cfg* cfg, const uint8_t* d_heaps, int n_tb,
+                      int32_t* d_acc, int accumulate, uint32_t* d_wrap);
+
 /* Image: replaces k_diff + k_grid_cube + cufftExecC2C + the host fftshift/Re/min-max
  * (src/rx_host_corr.cu:74-85,205-206,219-235). Grids the integration's V by baseline
  * (dy, dx) = ((qa-qb) mod ng, (pa-pb) mod ng) in exact integer arithmetic, runs a forward,
@@ -130,9 +138,14 @@ int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, 
 
 /* Launch tuning (the autotuner's search space), per array size class n_elem:
  *   slots     time split of one correlator call into int32 partial sums (1..8)
- *   prefetch  register prefetch stages of the correlator's 32-sample time loop (1, 2 or 4)
+ *   prefetch  correlator time-loop pipeline: 1 = register staging one 32-sample block ahead,
+ *             2 or 4 = direct global->LDS loads, that many blocks per barrier
  *   ct_chan   corner-turn channels per workgroup (8, 16 or 32)
- * 0 in a field means automatic. The table is process-wide. slots changes the accumulator
+ *   staging   how contexts feed the correlator: 1 = straight from the heaps (the corner-turn
+ *             is fused into the correlator's loads, mixer_xcorr_heaps), 2 = a separate
+ *             corner-turn pass into tiles (mixer_corner_turn + mixer_xcorr)
+ * 0 in a field means automatic: staging 1 with prefetch 4 for arrays of <= 256 elements,
+ * staging 2 with prefetch 1 above (the measured faster choices on MI355X). The table is process-wide. slots changes the accumulator
  * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
  * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
  * src/cgemm.cuh:18-24). */
@@ -140,13 +153,14 @@ typedef struct mixer_tuning {
     int32_t slots;
     int32_t prefetch;
     int32_t ct_chan;
-    int32_t reserved;
+    int32_t staging;
 } mixer_tuning;
 int mixer_set_tuning(int n_elem, const mixer_tuning* t);
 int mixer_get_tuning(const mixer_array_cfg* cfg, mixer_tuning* t); /* the plan in effect */
-/* Autotuner: times corner-turn + correlate + integrate + image of one integration of cfg on
- * synthetic device data for every (slots, prefetch) candidate, then every ct_chan, on device
- * `device`; stores the fastest in the tuning table and returns it in *best. */
+/* Autotuner: times (corner-turn +) correlate + integrate + image of one integration of cfg on
+ * synthetic device data for every (staging, slots, prefetch) candidate, then every ct_chan if
+ * the separate corner-turn won, on device `device`; stores the fastest in the tuning table and
+ * returns it in *best. */
 int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuning* best);
 
 /* Plain per-channel-launch form (baseline): one corner-turn launch and one full-square GEMM
@@ -225,7 +239,7 @@ void* mixer_compute_stream(mixer_ctx* ctx);
 /* ---- batched correlation of many independent arrays --------------------------------------
  * (no reference counterpart: the reference correlates one fixed array per process). A batch
  * holds n_jobs arrays of any supported geometry; each run correlates one integration
- * (cfg.k_integ samples) of every job with one corner-turn launch, ONE persistent correlator
+ * (cfg.k_integ samples) of every job with (optionally) one corner-turn launch, ONE persistent correlator
  * launch whose workgroups pull (job, channel group, super-block, time slot) work items from an
  * atomic counter (longest first), one integrate + gridding launch and one image launch. The
  * launch sequence is captured once as a hipGraph and replayed. Tuning:
@@ -233,13 +247,15 @@ void* mixer_compute_stream(mixer_ctx* ctx);
  *              0 = as many as fit (8 waves), 1 = one channel per workgroup
  *   tb_target  32-sample blocks per work item when a job is split in time (time slots are
  *              ceil(n_tb / tb_target), at most 8, only for jobs with fewer than 512 items)
- *   use_graph  1 = replay a captured hipGraph, 0 = plain launches on the batch stream */
+ *   use_graph  1 = replay a captured hipGraph, 0 = plain launches on the batch stream
+ *   staging    1 = the correlator stages straight from the heaps (no corner-turn launch),
+ *              2 = a separate corner-turn launch into tiles; 0 = automatic (1) */
 typedef struct mixer_batch mixer_batch;
 typedef struct mixer_batch_tuning {
     int32_t pack;
     int32_t tb_target;
     int32_t use_graph;
-    int32_t reserved;
+    int32_t staging;
 } mixer_batch_tuning;
 typedef struct mixer_batch_info {
     int32_t n_jobs;

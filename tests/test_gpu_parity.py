@@ -346,3 +346,105 @@ def test_autotune_sets_a_valid_plan():
         assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
     finally:
         M.lib().mixer_set_tuning(128, M.C.byref(M.Tuning(0, 0, 0, 0)))
+
+
+# ---------------------------------------------------------------- fused corner-turn + correlate
+@pytest.mark.parametrize("N,C,n_tb,parts", [
+    (256, 5, 8, 1), (256, 3, 12, 3), (32, 3, 3, 1), (64, 9, 4, 1), (96, 2, 2, 1), (160, 3, 5, 1),
+    (224, 2, 3, 1), (384, 2, 2, 1), (512, 3, 3, 1), (1024, 1, 2, 1), (640, 1, 4, 2),
+])
+def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
+    """The correlator staging straight from the heaps gives the CPU correlator's V, for every
+    pipeline variant and time split."""
+    log(f"xcorr_heaps N={N} C={C} n_tb={n_tb} parts={parts}")
+    bb = rand_heaps(n_tb, N, C)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    per = n_tb // parts
+    try:
+        for slots in (1, 2, 8):
+            for pf in (1, 2, 4):
+                if slots > per or (slots == 8 and pf != 1):
+                    continue
+                M.set_tuning(N, slots, pf)
+                cfg = cfg_for(N, C, 32 * per)
+                d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+                for p in range(parts):
+                    d_bb = M.DeviceBuffer.from_array(np.ascontiguousarray(bb[p * per:(p + 1) * per]))
+                    M.xcorr_heaps(cfg, d_bb, per, d_acc, accumulate=p > 0)
+                vre, vim = M.unpack_acc(d_acc.download(np.int32, (M.acc_bytes(cfg) // 4,)), cfg)
+                assert np.array_equal(vre, ore) and np.array_equal(vim, oim), (slots, pf)
+    finally:
+        M.set_tuning(N)
+
+
+@pytest.mark.parametrize("N,C,pf", [(256, 4, 1), (256, 4, 4), (96, 3, 2), (512, 2, 1), (512, 2, 2)])
+def test_xcorr_heaps_counts_imaginary_minus128(N, C, pf):
+    try:
+        M.set_tuning(N, prefetch=pf)
+        _check_minus128(N, C)
+    finally:
+        M.set_tuning(N)
+
+
+def _check_minus128(N, C):
+    n_tb = 4
+    cfg = cfg_for(N, C, 32 * n_tb)
+    bb = rand_heaps(n_tb, N, C, -127, 127)
+    d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+    d_w = M.DeviceBuffer(4)
+    d_w.zero()
+    d_bb = M.DeviceBuffer.from_array(bb)
+    M.xcorr_heaps(cfg, d_bb, n_tb, d_acc, d_wrap=d_w)
+    assert d_w.download(np.uint32, (1,))[0] == 0
+    # a real -128 is harmless; an imaginary one (element 37, channel 1, sample 5) is counted
+    bb2 = bb.copy()
+    bb2[1, 37, 1 * 32 + 5] = 0x80
+    d_bb.upload(bb2)
+    M.xcorr_heaps(cfg, d_bb, n_tb, d_acc, d_wrap=d_w)
+    assert d_w.download(np.uint32, (1,))[0] == 0
+    bb2[1, 37, C * 32 + 1 * 32 + 5] = 0x80
+    d_bb.upload(bb2)
+    M.xcorr_heaps(cfg, d_bb, n_tb, d_acc, d_wrap=d_w)
+    assert d_w.download(np.uint32, (1,))[0] >= 1
+    # the lower triangle stays exact with the -128 present
+    nb = N // 32
+    a = d_acc.download(np.int32, (M.acc_bytes(cfg) // 4,)).reshape(-1, C, nb * (nb + 1) // 2, 2, 32, 32)
+    a = a.astype(np.int64).sum(0)
+    ore, oim = O.v_full_from_heaps(bb2, n_tb, N, C, 32)
+    t = 0
+    for I in range(nb):
+        for J in range(I + 1):
+            assert np.array_equal(a[:, t, 0], ore[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32])
+            assert np.array_equal(a[:, t, 1], oim[:, I * 32:(I + 1) * 32, J * 32:(J + 1) * 32])
+            t += 1
+
+
+@pytest.mark.parametrize("staging", [1, 2])
+def test_engine_staging_modes_agree(staging):
+    """Contexts correlate from the heaps (staging 1) or through a separate corner-turn pass
+    (staging 2); both give the exact V and count an imaginary -128."""
+    N, C = 256, 8
+    cfg = cfg_for(N, C, 256)
+    cfg.k_integ = 512
+    n_tb = 16
+    bb = rand_heaps(n_tb, N, C)
+    bb[3, 100, C * 32 + 7] = 0x80
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    d = M.DeviceBuffer.from_array(bb)
+    try:
+        M.set_tuning(N, staging=staging)
+        assert M.get_tuning(cfg).staging == staging
+        with M.Engine(cfg) as e:
+            e.token_wait(e.submit_device(d, n_tb))
+            e.sync()
+            v, idx = e.dump()
+            st = e.stats()
+        assert idx == 1 and st.wrap128 >= 1
+        nb = N // 32
+        for I in range(nb):  # lower triangle (exact even with the -128)
+            for J in range(I + 1):
+                sl = (slice(None), slice(I * 32, (I + 1) * 32), slice(J * 32, (J + 1) * 32))
+                assert np.array_equal(v[:, 0][sl], ore[sl].astype(np.float32))
+                assert np.array_equal(v[:, 1][sl], oim[sl].astype(np.float32))
+    finally:
+        M.set_tuning(N)
