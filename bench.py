@@ -88,6 +88,120 @@ def algorithmic(cfg):
     }
 
 
+def geometry(N):
+    """(grid_x, ng) of an N-element filled array: grid_x x N/grid_x elements, image of at least
+    twice the larger side."""
+    gx = {64: 8, 128: 16, 256: 16}.get(N, 32)
+    ng = max(32, 1 << int(np.ceil(np.log2(2 * max(gx, N // gx)))))
+    return gx, ng
+
+
+def batch_jobs(seed=0x4D495834, n_jobs=64):
+    """The batched workload: 64 arrays, elements uniform in {64,128,256,512,1024}, channels in
+    [16,256], samples per integration in [1024,16384] (multiples of 32)."""
+    rng = np.random.default_rng(seed)
+    cfgs = []
+    for _ in range(n_jobs):
+        N = int(rng.choice([64, 128, 256, 512, 1024]))
+        C = int(rng.integers(16, 257))
+        K = int(rng.integers(1024 // 32, 16384 // 32 + 1)) * 32
+        gx, ng = geometry(N)
+        cfgs.append(M.default_cfg(n_elem=N, grid_x=gx, n_chan=C, k_integ=K, k_update=K, ng=ng,
+                                  pkt_pitch=max(16384, 64 + 64 * C)))
+    return cfgs
+
+
+def bench_batch(reps=5):
+    cfgs = batch_jobs()
+    payload = [c.k_integ // 32 * c.n_elem * c.payload for c in cfgs]
+    bufs = [M.DeviceBuffer(p) for p in payload]
+    for j, d in enumerate(bufs):
+        M.fill_random(d, seed=0x4D495834 + j)
+    M.check(M.lib().mixer_device_sync())
+    L = M.lib()
+    ev0, ev1, ms = M.C.c_void_p(), M.C.c_void_p(), M.C.c_float(0)
+    M.check(L.mixer_event_create(M.C.byref(ev0)))
+    M.check(L.mixer_event_create(M.C.byref(ev1)))
+
+    def timed(b):
+        b.run(bufs)
+        b.sync()
+        s = b.stream()
+        M.check(L.mixer_event_record(ev0, s))
+        for _ in range(reps):
+            b.run(bufs)
+        M.check(L.mixer_event_record(ev1, s))
+        b.sync()
+        M.check(L.mixer_event_elapsed_ms(ev0, ev1, M.C.byref(ms)))
+        return ms.value / reps
+
+    with M.Batch(cfgs) as b:
+        info = b.info()
+        t_def = timed(b)
+        best = b.autotune(reps=2)
+        t_tuned = timed(b)
+        info = b.info()
+    tot = sum(payload)
+    t = min(t_def, t_tuned)
+    out = {"jobs": len(cfgs), "payload_bytes": tot, "device_bytes": int(info.device_bytes),
+           "work_items": int(info.n_items), "graph": bool(info.graph),
+           "ms_per_run_default": round(t_def, 4), "ms_per_run_tuned": round(t_tuned, 4),
+           "tuning": best.as_dict(),
+           "value": round(tot / (t * 1e-3) / 1e9, 2), "unit": "GB/s",
+           "mfma_pct_peak": round(100.0 * info.exec_ops / (t * 1e-3) / 1e12 / INT8_PEAK_TOPS, 2),
+           "workload": "64 arrays (seed 0x4D495834): elements in {64..1024}, channels 16..256, "
+                       "1024..16384 samples; one integration of every array per run (hipGraph replay)"}
+    log(f"[bench] batch: {len(cfgs)} jobs, {tot / 2**30:.1f} GiB per run, default {t_def:.2f} ms, "
+        f"tuned {t_tuned:.2f} ms {best.as_dict()} -> {out['value']:.0f} GB/s, MFMA {out['mfma_pct_peak']:.1f}%")
+    for d in bufs:
+        d.free()
+    return out
+
+
+def serve_arrays(rate_gbps, seed=0x4D495835, n=8, dump_hz=20.0, chunks_per_integ=8):
+    """8 arrays (256..1024 elements, 64..256 channels) on a shared sample clock; each integrates
+    1/dump_hz seconds of samples at the given aggregate rate, in chunks_per_integ chunks."""
+    rng = np.random.default_rng(seed)
+    geo = []
+    for _ in range(n):
+        N = int(rng.choice([256, 384, 512, 640, 768, 896, 1024]))
+        C = int(rng.integers(64, 257))
+        geo.append((N, C))
+    fs = rate_gbps * 1e9 / sum(2 * N * C for N, C in geo)
+    unit = 32 * chunks_per_integ
+    K = int(min(65504 // unit, max(1, round(fs / dump_hz / unit)))) * unit
+    cfgs = []
+    for N, C in geo:
+        gx, ng = geometry(N)
+        cfgs.append(M.default_cfg(n_elem=N, grid_x=gx, n_chan=C, k_integ=K, k_update=K // chunks_per_integ,
+                                  ng=ng, pkt_pitch=max(16384, 64 + 64 * C)))
+    return cfgs, fs
+
+
+def bench_serve(rates=(50, 100, 250, 500), duration_s=6.5, budget=200 << 30):
+    out = []
+    for r in rates:
+        cfgs, fs = serve_arrays(r)
+        with M.Server(budget) as S:
+            for c in cfgs:
+                S.admit(c)
+            rep = S.run(duration_s=duration_s, aggregate_gbps=float(r))
+            st = S.stats()
+        d = {"offered_gbps": r, "achieved_gbps": round(rep.achieved_gbps, 2),
+             "integrations": int(rep.integrations), "chunks": int(rep.chunks),
+             "late_submits": int(rep.late_submits), "p50_ms": round(rep.lat_p50_ms, 3),
+             "p99_ms": round(rep.lat_p99_ms, 3), "max_ms": round(rep.lat_max_ms, 3),
+             "samples_per_integration": cfgs[0].k_integ, "dump_hz": round(fs / cfgs[0].k_integ, 2),
+             "hbm_used_gib": round(st.hbm_used / 2**30, 1)}
+        log(f"[bench] serve {r} GB/s: achieved {d['achieved_gbps']} GB/s, {d['integrations']} dumps, "
+            f"p50 {d['p50_ms']} ms p99 {d['p99_ms']} ms (max {d['max_ms']}), late {d['late_submits']}")
+        out.append(d)
+    return {"arrays": [(c.n_elem, c.n_chan) for c in serve_arrays(50)[0]], "runs": out,
+            "latency": "arrival of an integration's last chunk -> its image cube on the host (steady clock)",
+            "workload": "8 arrays (seed 0x4D495835), Poisson chunk arrivals (1/8 integration each), "
+                        "inputs in device rings (NIC-to-HBM model), 20 Hz dumps + images"}
+
+
 def cpu_baseline(cfg, heaps, n_tb, budget_s=12.0):
     """The oracle's CPU correlator (full-square reference math, int32) on this host: whole
     integrations of all channels, repeated until about budget_s seconds of CPU work."""
@@ -122,6 +236,9 @@ def main():
     ap.add_argument("--latency-n", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} not supported)",
@@ -287,6 +404,9 @@ def main():
     except Exception as ex:  # reported, never fatal
         log(f"[bench] H2D measurement skipped: {ex}")
 
+    batch = None if args.no_batch else bench_batch()
+    serving = None if args.no_serve else bench_serve(duration_s=args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, heaps, n_tb)
@@ -312,6 +432,8 @@ def main():
         "latency": latency,
         "plain_gpu": plain,
         "h2d": h2d,
+        "batch": batch,
+        "serving": serving,
         "cpu_baseline": cpu,
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
     }

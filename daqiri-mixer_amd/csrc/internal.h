@@ -12,6 +12,8 @@ namespace mixer {
 
 int hip_status(hipError_t e);
 int plan_slots(const mixer_array_cfg& c);
+// contexts correlate straight from the heaps (no corner-turn pass) under the tuning in effect
+bool plan_fused(const mixer_array_cfg& c);
 std::vector<std::pair<int, int>> sblocks_for(int nb);
 int get_plan(const mixer_array_cfg& c, XcorrPlan* p);
 int get_twiddles(int ng, const float2** out);

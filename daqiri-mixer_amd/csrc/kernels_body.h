@@ -73,12 +73,18 @@ __device__ __forceinline__ void corner_turn_body(const uint8_t* __restrict__ hea
 __device__ __forceinline__ void grid_tile(const int (*T)[32][33], unsigned long long* G, int I,
                                           int J, int gs, int ng) {
     const int nsub = 32 / gs, per = 2 * gs - 1, nseg = nsub * nsub * per;
-    for (int sg = threadIdx.x; sg < nseg; sg += 256) {
+    // diagonals are summed in chunks of at most 8 elements (4 per diagonal at grid_x 32), so
+    // all 256 threads share the work; each chunk adds its partial sum to the cell
+    const int len = gs > 8 ? 8 : gs, nck = gs / len;
+    for (int w = threadIdx.x; w < nseg * nck; w += 256) {
+        const int sg = w / nck, k = w % nck;
         const int sub = sg / per, d = sg % per - (gs - 1);
         const int ri = sub / nsub, rj = sub % nsub;
         const int i0 = d > 0 ? d : 0, i1 = d > 0 ? gs : gs + d;
+        const int c0 = i0 + k * len, c1 = c0 + len < i1 ? c0 + len : i1;
+        if (c0 >= c1) continue;
         long long sr = 0, si = 0;
-        for (int ip = i0; ip < i1; ++ip) {
+        for (int ip = c0; ip < c1; ++ip) {
             sr += T[0][ri * gs + ip][rj * gs + ip - d];
             si += T[1][ri * gs + ip][rj * gs + ip - d];
         }
@@ -124,17 +130,27 @@ __device__ __forceinline__ void integrate_body(const int32_t* __restrict__ acc,
         for (int q = tid; q < 2 * nn; q += 256) Gp[q] = 0ull;
     }
     const int jend = write_v ? nb : I + 1;
-    for (int J = 0; J < jend; ++J) {
-        int vals[2][4];
+    // the slot-summed tile of column block J, this thread's 2 x 4 values
+    auto load = [&](int J, v4i* x) {
         const bool lower = J <= I;
         const int32_t* t = ac + (size_t)(lower ? I * (I + 1) / 2 + J : J * (J + 1) / 2 + I) * 2048 +
                            i * 32 + j0;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            v4i x = *(const v4i*)(t + p * 1024);
-            for (int sl = 1; sl < n_slots; ++sl) x += *(const v4i*)(t + sl * slot_stride + p * 1024);
-            vals[p][0] = x.x; vals[p][1] = x.y; vals[p][2] = x.z; vals[p][3] = x.w;
+            x[p] = *(const v4i*)(t + p * 1024);
+            for (int sl = 1; sl < n_slots; ++sl) x[p] += *(const v4i*)(t + sl * slot_stride + p * 1024);
         }
+    };
+    v4i nxt[2];
+    load(0, nxt);
+    for (int J = 0; J < jend; ++J) {
+        int vals[2][4];
+        const bool lower = J <= I;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            vals[p][0] = nxt[p].x; vals[p][1] = nxt[p].y; vals[p][2] = nxt[p].z; vals[p][3] = nxt[p].w;
+        }
+        if (J + 1 < jend) load(J + 1, nxt);  // in flight while this tile is gridded and stored
         if (!lower || G) {
 #pragma unroll
             for (int p = 0; p < 2; ++p)

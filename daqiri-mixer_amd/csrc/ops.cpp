@@ -64,6 +64,11 @@ int plan_slots(const mixer_array_cfg& c) {
     return slots;
 }
 
+bool plan_fused(const mixer_array_cfg& c) {
+    const mixer_tuning t = tuning_for(c.n_elem);
+    return t.staging == 1 ? true : t.staging == 2 ? false : c.n_elem / 32 <= 8;
+}
+
 // Super-blocks (I0, J0) of the BLOCKED correlator: I0 % 4 == 0, J0 % 8 == 0, J0 <= I0 + 3.
 std::vector<std::pair<int, int>> sblocks_for(int nb) {
     std::vector<std::pair<int, int>> v;
@@ -93,7 +98,7 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     // automatic choices, from timings of whole integrations on MI355X: FOLD arrays correlate
     // fastest straight from the heaps with 4-block DMA stages; BLOCKED arrays reuse each
     // element block across many super-blocks and run faster from corner-turned tiles
-    p->fused = t.staging == 1 ? 1 : t.staging == 2 ? 0 : (nb <= 8 ? 1 : 0);
+    p->fused = plan_fused(c) ? 1 : 0;
     p->prefetch = t.prefetch > 0 ? t.prefetch : (p->fused && nb <= 8 ? 4 : 1);
     p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
     p->n_tasks = 0;

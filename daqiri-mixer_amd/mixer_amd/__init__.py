@@ -71,6 +71,25 @@ class BatchInfo(C.Structure):
                 ("payload_bytes", C.c_uint64), ("exec_ops", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("hbm_budget", "hbm_used", "admitted", "rejected", "chunks",
+                                          "bytes_in", "integrations", "lat_n")] + \
+               [(n, C.c_double) for n in ("lat_p50_ms", "lat_p99_ms", "lat_max_ms", "lat_mean_ms")]
+
+
+class ServeSpec(C.Structure):
+    _fields_ = [("duration_s", C.c_double), ("aggregate_gbps", C.c_double), ("seed", C.c_uint64)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("chunks", "integrations", "late_submits")] + \
+               [(n, C.c_double) for n in ("offered_gbps", "achieved_gbps", "wall_s", "lat_p50_ms",
+                                          "lat_p99_ms", "lat_max_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Sky(C.Structure):
     _fields_ = [("planet_l", C.c_double), ("planet_m", C.c_double), ("sigma", C.c_double),
                 ("star_amp", C.c_double), ("planet_scale", C.c_double), ("nch_gen", C.c_int32),
@@ -131,6 +150,20 @@ SIGNATURES = [
     ("mixer_batch_acc", I32, [P, I32, P, P]),
     ("mixer_batch_autotune", I32, [P, I32, P]),
     ("mixer_fill_random", I32, [P, P, SZ, U64]),
+    ("mixer_server_create", I32, [I32, U64, I32, P]),
+    ("mixer_server_destroy", I32, [P]),
+    ("mixer_server_footprint", U64, [P]),
+    ("mixer_server_admit", I32, [P, P, P]),
+    ("mixer_server_release", I32, [P, I32]),
+    ("mixer_server_input", I32, [P, I32, P, P, P]),
+    ("mixer_server_submit", I32, [P, I32, P, I32, U64, P]),
+    ("mixer_server_token_wait", I32, [P, I32, U64]),
+    ("mixer_server_drain", I32, [P]),
+    ("mixer_server_get_stats", I32, [P, P]),
+    ("mixer_server_reset_latency", None, [P]),
+    ("mixer_server_read_image", I32, [P, I32, P, P, P, P]),
+    ("mixer_server_dump", I32, [P, I32, P, P]),
+    ("mixer_serve_run", I32, [P, P, P]),
     ("mixer_set_device", I32, [I32]),
     ("mixer_dev_alloc", I32, [SZ, P]),
     ("mixer_dev_free", I32, [P]),
@@ -487,6 +520,102 @@ class Batch:
         t = BatchTuning()
         check(lib().mixer_batch_autotune(self.h, reps, C.byref(t)), "mixer_batch_autotune")
         return t
+
+
+# ---- serving loop ---------------------------------------------------------------------------------
+def server_footprint(cfg) -> int:
+    return int(lib().mixer_server_footprint(C.byref(cfg)))
+
+
+class Server:
+    """Arrays admitted under an HBM budget; chunks correlated as they arrive; per-integration
+    images dumped to pinned host memory with arrival-to-dump latency."""
+
+    def __init__(self, hbm_budget: int, device: int = 0, n_streams: int = 0):
+        h = C.c_void_p()
+        check(lib().mixer_server_create(device, int(hbm_budget), n_streams, C.byref(h)), "mixer_server_create")
+        self.h = h
+        self.cfgs = {}
+
+    def close(self):
+        if self.h:
+            check(lib().mixer_server_destroy(self.h), "mixer_server_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def admit(self, cfg) -> int:
+        """Array id; raises MixerError(MIXER_ERR_BUDGET) when it does not fit."""
+        i = C.c_int(0)
+        check(lib().mixer_server_admit(self.h, C.byref(cfg), C.byref(i)), "mixer_server_admit")
+        self.cfgs[i.value] = cfg
+        return i.value
+
+    def try_admit(self, cfg):
+        i = C.c_int(0)
+        st = lib().mixer_server_admit(self.h, C.byref(cfg), C.byref(i))
+        if st == ERR_BUDGET:
+            return None
+        check(st, "mixer_server_admit")
+        self.cfgs[i.value] = cfg
+        return i.value
+
+    def release(self, aid: int):
+        check(lib().mixer_server_release(self.h, aid), "mixer_server_release")
+        self.cfgs.pop(aid, None)
+
+    def input(self, aid: int):
+        p, n, cb = C.c_void_p(), C.c_int(0), C.c_size_t(0)
+        check(lib().mixer_server_input(self.h, aid, C.byref(p), C.byref(n), C.byref(cb)), "mixer_server_input")
+        return p.value, n.value, cb.value
+
+    def submit(self, aid: int, d_ptr, n_tb: int, arrival_ns: int = 0) -> int:
+        tok = C.c_uint64(0)
+        addr = d_ptr.ptr.value if isinstance(d_ptr, DeviceBuffer) else int(d_ptr)
+        check(lib().mixer_server_submit(self.h, aid, C.c_void_p(addr), n_tb, arrival_ns, C.byref(tok)),
+              "mixer_server_submit")
+        return tok.value
+
+    def token_wait(self, aid: int, tok: int):
+        check(lib().mixer_server_token_wait(self.h, aid, tok), "mixer_server_token_wait")
+
+    def drain(self):
+        check(lib().mixer_server_drain(self.h), "mixer_server_drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        check(lib().mixer_server_get_stats(self.h, C.byref(s)), "mixer_server_get_stats")
+        return s
+
+    def reset_latency(self):
+        lib().mixer_server_reset_latency(self.h)
+
+    def read_image(self, aid: int):
+        c = self.cfgs[aid]
+        img = np.empty((c.n_chan, c.ng, c.ng), np.float32)
+        vmax = np.empty(c.n_chan, np.float32)
+        vmin = np.empty(c.n_chan, np.float32)
+        idx = C.c_uint64(0)
+        check(lib().mixer_server_read_image(self.h, aid, ptr(img), ptr(vmax), ptr(vmin), C.byref(idx)),
+              "mixer_server_read_image")
+        return img, vmax, vmin, idx.value
+
+    def dump(self, aid: int):
+        c = self.cfgs[aid]
+        out = np.empty((c.n_chan, 2, c.n_elem, c.n_elem), np.float32)
+        idx = C.c_uint64(0)
+        check(lib().mixer_server_dump(self.h, aid, ptr(out), C.byref(idx)), "mixer_server_dump")
+        return out, idx.value
+
+    def run(self, duration_s: float, aggregate_gbps: float, seed: int = 0x4D495835) -> ServeReport:
+        spec = ServeSpec(duration_s, aggregate_gbps, seed)
+        rep = ServeReport()
+        check(lib().mixer_serve_run(self.h, C.byref(spec), C.byref(rep)), "mixer_serve_run")
+        return rep
 
 
 def corr32_bytes(cfg) -> int:

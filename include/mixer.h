@@ -182,7 +182,7 @@ typedef struct mixer_stats {
     uint64_t imissed;       /* time blocks dropped: no free slot (src/rx_host_corr.cu:140) */
     uint64_t misalign;      /* submissions whose first seq % n_elem != 0 (src/rx_host_corr.cu:135) */
     uint64_t seq_gaps;      /* time blocks dropped because a seq inside them was missing */
-    uint64_t wrap128;       /* corner-turn workgroups that saw an imaginary -128 */
+    uint64_t wrap128;       /* corner-turn or correlator workgroups that saw an imaginary -128 */
     double last_dump_latency_ms; /* arrival of the integration's last heap -> V/image ready */
 } mixer_stats;
 
@@ -289,6 +289,75 @@ int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best);
 /* Fills n bytes of device memory with seeded uniform int8 values in [-127, 127]. */
 int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed);
+
+/* ---- serving loop: many arrays, integrations admitted as their data arrives ----------------
+ * No reference counterpart (the reference serves one array per process, its gpu_thread,
+ * src/rx_host_corr.cu:143-175). A server holds the arrays admitted under an HBM budget; for
+ * each it keeps the int32 accumulator, the f32 V cube (double-buffered), the image grid and an
+ * input ring resident on the device. mixer_server_submit hands over one chunk of an array's
+ * header-free heaps (a whole number of V-updates, device memory, e.g. written there by a NIC)
+ * with the arrival time of its last heap; the server correlates it on the array's stream and,
+ * when the chunk completes an integration, integrates + images it and copies the image cube to
+ * pinned host memory. A completion thread timestamps each such dump: latency = dump complete on
+ * the host - arrival of the integration's last chunk (steady clock).
+ * Arrays are spread over n_streams compute streams (0 = 4). */
+typedef struct mixer_server mixer_server;
+typedef struct mixer_server_stats {
+    uint64_t hbm_budget;    /* bytes */
+    uint64_t hbm_used;      /* bytes held by admitted arrays */
+    uint64_t admitted;      /* arrays admitted so far */
+    uint64_t rejected;      /* admissions refused for budget */
+    uint64_t chunks;        /* chunks submitted */
+    uint64_t bytes_in;      /* payload bytes submitted */
+    uint64_t integrations;  /* dumps completed on the host */
+    uint64_t lat_n;         /* latency samples */
+    double lat_p50_ms, lat_p99_ms, lat_max_ms, lat_mean_ms;
+} mixer_server_stats;
+
+int mixer_server_create(int device, uint64_t hbm_budget, int n_streams, mixer_server** out);
+int mixer_server_destroy(mixer_server* srv);
+/* Device bytes an admitted array of this geometry holds (its admission cost). */
+uint64_t mixer_server_footprint(const mixer_array_cfg* cfg);
+/* Admit an array (cfg.k_update = samples per chunk unit, cfg.k_integ = samples per
+ * integration); MIXER_ERR_BUDGET if its footprint does not fit the remaining budget. */
+int mixer_server_admit(mixer_server* srv, const mixer_array_cfg* cfg, int* array_id);
+int mixer_server_release(mixer_server* srv, int array_id);
+/* The array's device input ring: n_chunks chunks of chunk_bytes (= k_update/32 time blocks of
+ * header-free heaps each) that a producer may fill and then submit. */
+int mixer_server_input(mixer_server* srv, int array_id, uint8_t** d_ring, int* n_chunks,
+                       size_t* chunk_bytes);
+/* Submit n_tb time blocks (a multiple of k_update/32) at device address d_heaps. The memory is
+ * read asynchronously: *token (optional) completes when it may be reused
+ * (mixer_server_token_wait). */
+int mixer_server_submit(mixer_server* srv, int array_id, const uint8_t* d_heaps, int n_tb,
+                        uint64_t arrival_ns, uint64_t* token);
+int mixer_server_token_wait(mixer_server* srv, int array_id, uint64_t token);
+/* Wait until everything submitted is dumped. */
+int mixer_server_drain(mixer_server* srv);
+int mixer_server_get_stats(mixer_server* srv, mixer_server_stats* st);
+void mixer_server_reset_latency(mixer_server* srv);
+/* Last dumped integration of an array: the host image copy (img [C][ng][ng], vmax/vmin [C]),
+ * V from device (host_v [C][2][N][N]) and its integration count. */
+int mixer_server_read_image(mixer_server* srv, int array_id, float* img, float* vmax, float* vmin,
+                            uint64_t* integ_index);
+int mixer_server_dump(mixer_server* srv, int array_id, float* host_v, uint64_t* integ_index);
+
+/* Load generator: Poisson arrivals of chunks (k_update samples) from every admitted array for
+ * duration_s seconds at aggregate_gbps of payload, shared sample clock (each array's rate is
+ * proportional to its bytes per sample), data from the arrays' input rings (filled with seeded
+ * int8 noise at admission). Submits each chunk at its arrival time from the calling thread and
+ * reports throughput and latency of the dumps that completed. */
+typedef struct mixer_serve_spec {
+    double duration_s;
+    double aggregate_gbps;
+    uint64_t seed;
+} mixer_serve_spec;
+typedef struct mixer_serve_report {
+    uint64_t chunks, integrations, late_submits; /* late: submitted > 1 ms after arrival */
+    double offered_gbps, achieved_gbps, wall_s;
+    double lat_p50_ms, lat_p99_ms, lat_max_ms;
+} mixer_serve_report;
+int mixer_serve_run(mixer_server* srv, const mixer_serve_spec* spec, mixer_serve_report* rep);
 
 /* ---- HIP plumbing for hosts without a HIP binding of their own -------------------------------
  * (no reference counterpart: the reference calls the CUDA runtime directly). The Python tests
