@@ -414,6 +414,14 @@ def main():
 
     d = kern[dom]
     peak = HBM_PEAK_GBS
+    # HBM bytes per correlator launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # this same command (scaled by a calibration copy); only valid for the launch shape profiled
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "xcorr_fold8_256el_128ch_8192s_traffic.json")
+    tun = M.get_tuning(cfg)
+    if args.config == "cfg2" and dom == "xcorr" and tun.staging == 1 and tun.prefetch == 4 and os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = int(json.load(fh)["traffic_bytes"])
     out = {
         "metric": "correlated input GB/s + int8 MFMA % peak, p99 dump latency; 256-elt×128-ch",
         "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -427,7 +435,7 @@ def main():
         "x_400gbe_equivalent": round(value / GBE400_PAYLOAD_GBS, 2),
         "mfma_pct_peak": round(mfma_pct, 2),
         "roofline": {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"], "peak": peak,
-                     "unit": "GB/s", "frac": round(d["achieved"] / peak, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(d["achieved"] / peak, 4), "traffic": traffic},
         "kernels": kern,
         "latency": latency,
         "plain_gpu": plain,
