@@ -63,14 +63,17 @@ void destroy_graph(mixer_batch* b) {
     b->graph = nullptr;
 }
 
-bool fused(const mixer_batch* b) { return b->tun.staging != 2; }
+// staging 0 (automatic): arrays of <= 8 element blocks straight from the heaps, larger ones
+// (whose element blocks are re-read by many super-blocks) from corner-turned tiles
+bool job_fused(const mixer_batch* b, int nb) {
+    return b->tun.staging == 1 ? true : b->tun.staging == 2 ? false : nb <= 8;
+}
 
 int enqueue(mixer_batch* b) {
     MIXER_HIP(launch_batch_reset(b->s, b->d_counter));
-    if (!fused(b))
+    if (b->ct_total > 0)
         MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
-    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs,
-                                 fused(b) ? 1 : 0));
+    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs));
     MIXER_HIP(launch_batch_integrate(b->s, b->d_jobs, (int)b->jobs.size(), b->int_total, b->max_ng));
     MIXER_HIP(launch_batch_image(b->s, b->d_jobs, (int)b->jobs.size(), b->img_total, b->max_ng));
     return MIXER_OK;
@@ -102,7 +105,7 @@ int plan(mixer_batch* b) {
                 const auto sb = sblocks_for(nb);
                 for (int ch = 0; ch < c.n_chan; ++ch)
                     for (size_t q = 0; q < sb.size(); ++q) {
-                        const int tiles = sb[q].second + 7 <= sb[q].first ? 32 : 20;
+                        const int tiles = sb[q].second < 0 ? 36 : sb[q].second + 8 <= sb[q].first ? 32 : 10;
                         items.push_back({make_int4((int)j, ch, (int)q, sl), kb * tiles});
                     }
             }
@@ -175,7 +178,8 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
         // accumulator capacity: the most time slots any tuning candidate (tb_target >= 64) uses
         const int cap = std::min(slots_for(c, 0, 64), 8);
         b->slot_cap.push_back(cap);
-        if (!fused(b)) MIXER_TRY(balloc(b, (void**)&J.tiles, mixer_tiles_bytes(&c, n_tb)));
+        J.fused = job_fused(b, nb) ? 1 : 0;
+        if (!J.fused) MIXER_TRY(balloc(b, (void**)&J.tiles, mixer_tiles_bytes(&c, n_tb)));
         MIXER_TRY(balloc(b, (void**)&J.acc, (size_t)cap * c.n_chan * NT * 2048 * sizeof(int32_t)));
         MIXER_TRY(balloc(b, (void**)&J.v, (size_t)c.n_chan * 2 * c.n_elem * c.n_elem * sizeof(float)));
         const size_t gbytes = (size_t)c.n_chan * c.ng * c.ng * 16;
@@ -189,7 +193,8 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
             J.sblocks = (const int2*)p.d_tasks;
             J.n_sb = p.n_tasks;
         }
-        J.ct_wg0 = ct; ct += n_tb * nb * ((c.n_chan + 7) / 8);
+        J.ct_wg0 = ct;  // corner-turn workgroups only for jobs correlated from tiles
+        if (!J.fused) ct += n_tb * nb * ((c.n_chan + 7) / 8);
         J.int_wg0 = in; in += nb * c.n_chan;
         J.img_wg0 = im; im += c.n_chan;
         b->max_ng = std::max(b->max_ng, c.ng);

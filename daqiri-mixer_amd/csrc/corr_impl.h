@@ -5,7 +5,9 @@
 //   k_xcorr_blk<NB,V>   arrays of NB = 12..32 blocks (N = 384..1024, N % 128 == 0)
 // V selects the time-loop pipeline (the plan's "prefetch" field):
 //   V = 1   one 32-sample block per barrier, staged global -> registers -> LDS one block ahead;
-//   V = K>1 K blocks per barrier (FOLD: K = V, BLOCKED: K = min(V, 3)), staged by direct
+//   V = 3   the same with two register sets, loads issued two blocks ahead (more bytes in
+//           flight per CU for the latency-bound gather straight from the heaps);
+//   V = 2,4 K = V blocks per barrier (BLOCKED: K = min(V, 3)), staged by direct
 //           global -> LDS loads (global_load_lds_dwordx4) one stage ahead into two separately
 //           declared LDS buffers, so the barrier and LDS-latency bubble at each stage boundary
 //           is paid once per K blocks and no staging registers are needed.
@@ -19,7 +21,7 @@ namespace mixer {
 
 template <int V, int NBLK>
 struct StageOf {
-    static constexpr int KB = V == 1 ? 1 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
+    static constexpr int KB = (V == 1 || V == 3) ? 1 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
 };
 
 // The MFMAs of one 32-sample block for tiles (RA, [A0,A1)) u (RB, [B0,B1)) from the LDS slab L
@@ -60,7 +62,60 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
                                           v16i* ar, v16i* ai, v16i* br, v16i* bi, int ltid,
                                           bool chk, unsigned& saw) {
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
-    if constexpr (V == 1) {
+    if constexpr (V == 3) {
+        // two register sets: set s0 holds even blocks, s1 odd blocks; the loop body is written
+        // out for both parities so both sets keep fixed registers (the wait before each LDS
+        // store then covers only the older set's loads)
+        auto ident = [](int b) { return b; };
+        using St = Stager<NE, NBLK, NTH>;
+        St s0, s1;
+        auto gload = [&](St& st, const int8_t* sl) {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
+                    const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                    st.r[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
+                }
+            }
+        };
+        auto check = [&](const St& st) {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if ((St::CHUNKS % NTH == 0 || u < St::CHUNKS) && ((u >> 6) & 1))
+                    saw |= has80(st.r[i].x) | has80(st.r[i].y) | has80(st.r[i].z) | has80(st.r[i].w);
+            }
+        };
+        gload(s0, src);
+        gload(s1, src + (n_kb > 1 ? m.tb : 0));
+        if (chk) check(s0);
+        s0.store(lds0, ident, ltid);
+        __syncthreads();
+        const int8_t* nx = src + 2 * m.tb;
+        int kb = 0;
+        for (; kb + 2 <= n_kb; kb += 2) {
+            // even step: block kb in lds0, s1 = kb+1, load kb+2 into s0
+            if (kb + 2 < n_kb) gload(s0, nx);
+            block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds0, lrow, kg, ar, ai, br, bi);
+            if (chk) check(s1);
+            s1.store(lds1, ident, ltid);
+            __syncthreads();
+            // odd step: block kb+1 in lds1, s0 = kb+2, load kb+3 into s1
+            if (kb + 3 < n_kb) gload(s1, nx + m.tb);
+            block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds1, lrow, kg, ar, ai, br, bi);
+            if (kb + 2 < n_kb) {
+                if (chk) check(s0);
+                s0.store(lds0, ident, ltid);
+            }
+            __syncthreads();
+            nx += 2 * m.tb;
+        }
+        if (kb < n_kb) {  // odd tail: block kb in lds0
+            block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds0, lrow, kg, ar, ai, br, bi);
+            __syncthreads();
+        }
+    } else if constexpr (V == 1) {
         auto ident = [](int b) { return b; };
         using St = Stager<NE, NBLK, NTH>;
         St st;
@@ -144,13 +199,15 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
     }
 }
 
-// FOLD wave program: tiles (RA, [A0,A1)) u (RB, [B0,B1)) of the lower triangle of NB blocks;
-// acc points at the channel's tile-packed accumulator (tile t = I(I+1)/2 + J, 2048 ints).
+// FOLD wave program: tiles (RA, [A0,A1)) u (RB, [B0,B1)) of the lower triangle of NB blocks
+// starting at element block d0 (0 for a whole array; the diagonal triangles of the BLOCKED
+// decomposition use d0 = 8k); acc points at the channel's tile-packed accumulator (tile
+// t = I(I+1)/2 + J, 2048 ints).
 template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
 __device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const XSrc& m,
                                           int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
                                           int n_kb, int accumulate, int ltid, bool chk,
-                                          unsigned& saw) {
+                                          unsigned& saw, int d0) {
     constexpr int NA = A1 - A0;
     constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
@@ -160,18 +217,19 @@ __device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const 
 #pragma unroll
     for (int j = 0; j < NBT; ++j) { br[j] = (v16i){0}; bi[j] = (v16i){0}; }
     corr_wave<V, NB * 32, NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, lds0, lds1, n_kb,
-                                                           [](int b) { return b; }, ar, ai, br,
-                                                           bi, ltid, chk, saw);
+                                                           [=](int b) { return d0 + b; }, ar, ai,
+                                                           br, bi, ltid, chk, saw);
+    const int IA = d0 + RA, IB = d0 + RB;
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
-        int32_t* t = acc + (size_t)(RA * (RA + 1) / 2 + A0 + j) * 2048;
+        int32_t* t = acc + ((size_t)IA * (IA + 1) / 2 + d0 + A0 + j) * 2048;
         store_tile(ar[j], t, lrow, kg, accumulate);
         store_tile(ai[j], t + 1024, lrow, kg, accumulate);
     }
     if constexpr (RB >= 0) {
 #pragma unroll
         for (int j = 0; j < NBT; ++j) {
-            int32_t* t = acc + (size_t)(RB * (RB + 1) / 2 + B0 + j) * 2048;
+            int32_t* t = acc + ((size_t)IB * (IB + 1) / 2 + d0 + B0 + j) * 2048;
             store_tile(br[j], t, lrow, kg, accumulate);
             store_tile(bi[j], t + 1024, lrow, kg, accumulate);
         }
@@ -195,28 +253,28 @@ struct FoldLayout {
 template <int V, int NB, int W, int SPLIT_FROM = 6>
 __device__ __forceinline__ void fold_run(const int8_t* src, const XSrc& m, int32_t* acc,
                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                         int ltid, bool chk, unsigned& saw) {
+                                         int ltid, bool chk, unsigned& saw, int d0) {
     using Lay = FoldLayout<NB, SPLIT_FROM>;
     constexpr int NTH = Lay::NTH;
     if constexpr (!Lay::SPLIT) {
         constexpr int R1 = W, R2 = NB - 1 - W;
         if constexpr (R1 == R2)
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
         else
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
     } else {
         constexpr int p = W % Lay::P;
         constexpr int R1 = p, R2 = NB - 1 - p;
         constexpr int T = (R1 == R2) ? R1 + 1 : R1 + R2 + 2;
         constexpr int S = (T + 1) / 2;
         if constexpr (W < Lay::P) {
-            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
         } else if constexpr (R1 == R2) {
-            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
         } else if constexpr (S > R2) {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
         } else {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
         }
     }
 }
@@ -232,12 +290,13 @@ struct FoldLds {
 template <int NB, int V, int SPLIT_FROM = 6>
 __device__ __forceinline__ void fold_item(const int8_t* src, const XSrc& m, int32_t* acc,
                                           uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                          int ltid, bool chk, unsigned& saw) {
+                                          int ltid, bool chk, unsigned& saw, int d0 = 0) {
     const int w = ltid >> 6;
     static_for<0, FoldLayout<NB, SPLIT_FROM>::WAVES>([&](auto Wc) {
         constexpr int W = decltype(Wc)::value;
         if (w == W)
-            fold_run<V, NB, W, SPLIT_FROM>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw);
+            fold_run<V, NB, W, SPLIT_FROM>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk,
+                                           saw, d0);
     });
 }
 
@@ -326,11 +385,18 @@ struct BlkLds {
     static constexpr int BYTES = StageOf<V, 12>::KB * 12 * 32 * 64;
 };
 
-// One super-block (I0, J0) of the BLOCKED correlator on 512 threads with local ids ltid.
+// One work item (I0, J0) of the BLOCKED correlator on 512 threads with local ids ltid:
+// J0 < 0 is the diagonal triangle of element blocks I0..I0+7, run with the 8-wave FOLD layout
+// (36 tiles, no idle wave); J0 >= 0 is the 4x8-tile super-block at rows I0.., columns J0.. .
 template <int NB, int V>
 __device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32_t* a,
                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
                                          int I0, int J0, int ltid, bool chk, unsigned& saw) {
+    static_assert(FoldLds<8, V>::BYTES <= BlkLds<V>::BYTES, "fold stage must fit the blocked LDS");
+    if (J0 < 0) {
+        fold_item<8, V>(src, m, a, lds0, lds1, n_kb, accumulate, ltid, chk, saw, I0);
+        return;
+    }
     auto gblk = [=](int s) {
         const int g = s < 4 ? I0 + s : J0 + (s - 4);
         return g < NB ? g : NB - 1;  // columns past the array only feed EMPTY waves

@@ -76,14 +76,14 @@ struct BatchJobDev {
     int n_elem, n_chan, n_tb, n_slots, grid_x, ng, nb, n_sb;
     int ct_wg0, int_wg0, img_wg0;  // first workgroup of this job in each batched launch
     int ch_per_item;               // FOLD: channels one correlator work item covers (>= 1)
+    int fused;                     // 1: correlate straight from the heaps, 0: from the tiles
 };
 
 // Correlator work item: (job, first channel, super-block index or 0, time slot)
 hipError_t launch_batch_reset(hipStream_t s, int* counter);
 hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
-// fused: stage from each job's heaps (no corner-turn launch) instead of its tiles
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs, int fused);
+                              int n_items, int* counter, int n_wgs);
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
                                   int max_ng);
 hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,

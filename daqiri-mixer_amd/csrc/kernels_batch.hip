@@ -183,7 +183,7 @@ constexpr int kBatchLds = 48 * 1024;
 // so the longest start first and the tail is made of short items.
 __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restrict__ jobs,
                                                      const int4* __restrict__ items, int n_items,
-                                                     int* __restrict__ counter, int fused) {
+                                                     int* __restrict__ counter) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[kBatchLds];
     __shared__ int item_sh;
     const int w8 = threadIdx.x >> 6;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
         if (kb1 > kb0) {  // uniform
             // One call site for every array size (a second inlined copy makes the allocator
             // spill): fold channels and BLOCKED super-blocks differ only in these parameters.
-            const XSrc m = fused ? xsrc_heaps(J.heaps, J.n_elem, J.n_chan)
+            const XSrc m = J.fused ? xsrc_heaps(J.heaps, J.n_elem, J.n_chan)
                                  : xsrc_tiles(J.tiles, J.n_elem, J.n_tb);
             int nblk, buf, ltid, nth, c, row0, col0;
             bool active = false, have = true;
@@ -224,14 +224,23 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
                 col0 = 4;  // identity block map
             } else {
                 const int2 sb = J.sblocks[w.z];
-                nblk = 12;
-                buf = 12 * 32 * 64;
                 nth = 512;
                 ltid = threadIdx.x;
                 c = w.y;
-                wt = blk_tiles_rt(sb.x, sb.y, w8, &active);
-                row0 = sb.x;
-                col0 = sb.y;
+                if (sb.y < 0) {  // diagonal triangle of blocks sb.x..sb.x+7: the 8-wave fold layout
+                    nblk = 8;
+                    buf = 8 * 32 * 64;
+                    wt = fold_tiles_rt(8, w8, &active);
+                    wt.ra_g += sb.x; wt.a0_g += sb.x; wt.rb_g += sb.x; wt.b0_g += sb.x;
+                    row0 = sb.x;
+                    col0 = sb.x + 4;
+                } else {
+                    nblk = 12;
+                    buf = 12 * 32 * 64;
+                    wt = blk_tiles_rt(sb.x, sb.y, w8, &active);
+                    row0 = sb.x;
+                    col0 = sb.y;
+                }
             }
             auto gblk = [=](int b) {
                 const int g = b < 4 ? row0 + b : col0 + (b - 4);
@@ -278,9 +287,8 @@ hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int 
 }
 
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs, int fused) {
-    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter,
-                       fused);
+                              int n_items, int* counter, int n_wgs) {
+    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter);
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
 // kernels_corr.hip -- dispatch of the correlator launch plan to the kernel variants
-// (corr_impl.h; instantiated per prefetch depth in corr_{fold,blk}_pf{1,2,4}.hip).
+// (corr_impl.h; instantiated per pipeline variant in corr_{fold,blk}_pf{1,2,3,4}.hip).
 #include "kernels.h"
 
 namespace mixer {
@@ -10,9 +10,11 @@ namespace mixer {
                                     int accumulate);
 MIXER_DECL(fold, 1)
 MIXER_DECL(fold, 2)
+MIXER_DECL(fold, 3)
 MIXER_DECL(fold, 4)
 MIXER_DECL(blk, 1)
 MIXER_DECL(blk, 2)
+MIXER_DECL(blk, 3)
 MIXER_DECL(blk, 4)
 #undef MIXER_DECL
 
@@ -24,6 +26,8 @@ hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const XSrc& src, int3
                         : launch_blk_1(s, p, src, acc, wrap, C, n_tb, accumulate);
     case 2: return fold ? launch_fold_2(s, p, src, acc, wrap, C, n_tb, accumulate)
                         : launch_blk_2(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 3: return fold ? launch_fold_3(s, p, src, acc, wrap, C, n_tb, accumulate)
+                        : launch_blk_3(s, p, src, acc, wrap, C, n_tb, accumulate);
     case 4: return fold ? launch_fold_4(s, p, src, acc, wrap, C, n_tb, accumulate)
                         : launch_blk_4(s, p, src, acc, wrap, C, n_tb, accumulate);
     default: return hipErrorInvalidValue;

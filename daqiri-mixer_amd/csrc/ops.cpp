@@ -69,11 +69,18 @@ bool plan_fused(const mixer_array_cfg& c) {
     return t.staging == 1 ? true : t.staging == 2 ? false : c.n_elem / 32 <= 8;
 }
 
-// Super-blocks (I0, J0) of the BLOCKED correlator: I0 % 4 == 0, J0 % 8 == 0, J0 <= I0 + 3.
+// Work items (I0, J0) of the BLOCKED correlator (nb % 4 == 0) covering the lower triangle
+// exactly once: the diagonal triangles of 8 element blocks (I0 = 8k, J0 = -1; 36 tiles on the
+// 8-wave FOLD layout), the full 4x8-tile super-blocks below them (I0 % 4 == 0, J0 % 8 == 0,
+// J0 + 8 <= 8 * (I0 / 8)), and for nb % 8 == 4 the last 4-block triangle as a super-block
+// (I0 = J0 = nb - 4, whose waves past the diagonal idle).
 std::vector<std::pair<int, int>> sblocks_for(int nb) {
     std::vector<std::pair<int, int>> v;
-    for (int I0 = 0; I0 < nb; I0 += 4)
-        for (int J0 = 0; J0 <= I0 + 3 && J0 < nb; J0 += 8) v.push_back({I0, J0});
+    const int q = nb / 8;
+    for (int d = 0; d < q; ++d) v.push_back({8 * d, -1});
+    for (int I0 = 8; I0 < nb; I0 += 4)
+        for (int J0 = 0; J0 + 8 <= (I0 / 8) * 8; J0 += 8) v.push_back({I0, J0});
+    if (nb % 8) v.push_back({8 * q, 8 * q});
     return v;
 }
 
@@ -96,10 +103,10 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     p->n_slots = plan_slots(c);
     const mixer_tuning t = tuning_for(c.n_elem);
     // automatic choices, from timings of whole integrations on MI355X: FOLD arrays correlate
-    // fastest straight from the heaps with 4-block DMA stages; BLOCKED arrays reuse each
-    // element block across many super-blocks and run faster from corner-turned tiles
+    // fastest straight from the heaps with loads issued two blocks ahead; BLOCKED arrays
+    // reuse each element block across many super-blocks and run faster from corner-turned tiles
     p->fused = plan_fused(c) ? 1 : 0;
-    p->prefetch = t.prefetch > 0 ? t.prefetch : (p->fused && nb <= 8 ? 4 : 1);
+    p->prefetch = t.prefetch > 0 ? t.prefetch : (p->fused && nb <= 8 ? 3 : 1);
     p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
     p->n_tasks = 0;
     p->d_tasks = nullptr;
@@ -315,7 +322,7 @@ int mixer_image(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, fl
 int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
     if (!t || n_elem < 32 || n_elem > 1024 || n_elem % 32) return MIXER_ERR_ARG;
     if (t->slots < 0 || t->slots > 8) return MIXER_ERR_ARG;
-    if (t->prefetch != 0 && t->prefetch != 1 && t->prefetch != 2 && t->prefetch != 4) return MIXER_ERR_ARG;
+    if (t->prefetch < 0 || t->prefetch > 4) return MIXER_ERR_ARG;
     if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
     if (t->staging < 0 || t->staging > 2) return MIXER_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_tune_mtx);

@@ -99,7 +99,7 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
     for (int staging : {1, 2}) {
         for (int slots : {1, 2, 4, 8}) {
             if (slots > n_tb) continue;
-            for (int pf : {1, 2, 4}) {
+            for (int pf : {1, 2, 3, 4}) {
                 t.slots = slots;
                 t.prefetch = pf;
                 t.staging = staging;

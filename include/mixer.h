@@ -139,12 +139,13 @@ int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, 
 /* Launch tuning (the autotuner's search space), per array size class n_elem:
  *   slots     time split of one correlator call into int32 partial sums (1..8)
  *   prefetch  correlator time-loop pipeline: 1 = register staging one 32-sample block ahead,
- *             2 or 4 = direct global->LDS loads, that many blocks per barrier
+ *             3 = register staging two blocks ahead, 2 or 4 = direct global->LDS loads, that
+ *             many blocks per barrier
  *   ct_chan   corner-turn channels per workgroup (8, 16 or 32)
  *   staging   how contexts feed the correlator: 1 = straight from the heaps (the corner-turn
  *             is fused into the correlator's loads, mixer_xcorr_heaps), 2 = a separate
  *             corner-turn pass into tiles (mixer_corner_turn + mixer_xcorr)
- * 0 in a field means automatic: staging 1 with prefetch 4 for arrays of <= 256 elements,
+ * 0 in a field means automatic: staging 1 with prefetch 3 for arrays of <= 256 elements,
  * staging 2 with prefetch 1 above (the measured faster choices on MI355X). The table is process-wide. slots changes the accumulator
  * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
  * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
@@ -249,7 +250,8 @@ void* mixer_compute_stream(mixer_ctx* ctx);
  *              ceil(n_tb / tb_target), at most 8, only for jobs with fewer than 512 items)
  *   use_graph  1 = replay a captured hipGraph, 0 = plain launches on the batch stream
  *   staging    1 = the correlator stages straight from the heaps (no corner-turn launch),
- *              2 = a separate corner-turn launch into tiles; 0 = automatic (1) */
+ *              2 = a separate corner-turn launch into tiles; 0 = automatic: 1 for arrays of
+ *              <= 256 elements, 2 above (fixed at creation) */
 typedef struct mixer_batch mixer_batch;
 typedef struct mixer_batch_tuning {
     int32_t pack;

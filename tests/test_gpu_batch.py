@@ -58,12 +58,12 @@ def check_job(b, j, cfg, bb, n_tb, images=True):
         assert np.array_equal(vmin, img.reshape(C, -1).min(1))
 
 
-@pytest.mark.parametrize("pack,use_graph", [(0, 1), (1, 1), (0, 0)])
-def test_batch_uneven_jobs_bit_exact(pack, use_graph):
-    log(f"batch pack={pack} graph={use_graph} jobs={JOBS}")
+@pytest.mark.parametrize("pack,use_graph,staging", [(0, 1, 0), (1, 1, 0), (0, 0, 0), (0, 1, 1), (1, 1, 2)])
+def test_batch_uneven_jobs_bit_exact(pack, use_graph, staging):
+    log(f"batch pack={pack} graph={use_graph} staging={staging} jobs={JOBS}")
     cfgs = [cfg_for(*j) for j in JOBS]
     t = M.batch_default_tuning()
-    t.pack, t.use_graph = pack, use_graph
+    t.pack, t.use_graph, t.staging = pack, use_graph, staging
     with M.Batch(cfgs, tuning=t) as b:
         info = b.info()
         assert info.n_jobs == len(JOBS) and info.graph == use_graph and info.n_items > 0

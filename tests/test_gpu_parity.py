@@ -314,7 +314,7 @@ def test_every_launch_variant_bit_exact(N, C):
     d_bb = M.DeviceBuffer.from_array(bb)
     try:
         for slots in (1, 2, 8):
-            for pf in (1, 2, 4):
+            for pf in (1, 2, 3, 4):
                 for ct in (8, 16, 32):
                     if ct != 32 and (slots, pf) != (2, 2):
                         continue
@@ -336,7 +336,8 @@ def test_autotune_sets_a_valid_plan():
     cfg = cfg_for(128, 16, 1024)
     try:
         best = M.autotune(cfg)
-        assert best.slots in (1, 2, 4, 8) and best.prefetch in (1, 2, 4) and best.ct_chan in (8, 16, 32)
+        assert best.slots in (1, 2, 4, 8) and best.prefetch in (1, 2, 3, 4) and best.ct_chan in (8, 16, 32)
+        assert best.staging in (1, 2)
         t = M.get_tuning(cfg)
         assert (t.slots, t.prefetch, t.ct_chan) == (best.slots, best.prefetch, best.ct_chan)
         bb = rand_heaps(32, 128, 16)
@@ -362,8 +363,8 @@ def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
     per = n_tb // parts
     try:
         for slots in (1, 2, 8):
-            for pf in (1, 2, 4):
-                if slots > per or (slots == 8 and pf != 1):
+            for pf in (1, 2, 3, 4):
+                if slots > per or (slots == 8 and pf not in (1, 3)):
                     continue
                 M.set_tuning(N, slots, pf)
                 cfg = cfg_for(N, C, 32 * per)
@@ -377,7 +378,8 @@ def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
         M.set_tuning(N)
 
 
-@pytest.mark.parametrize("N,C,pf", [(256, 4, 1), (256, 4, 4), (96, 3, 2), (512, 2, 1), (512, 2, 2)])
+@pytest.mark.parametrize("N,C,pf", [(256, 4, 1), (256, 4, 3), (256, 4, 4), (96, 3, 2), (160, 2, 3), (512, 2, 1),
+                                    (512, 2, 2), (512, 2, 3)])
 def test_xcorr_heaps_counts_imaginary_minus128(N, C, pf):
     try:
         M.set_tuning(N, prefetch=pf)
