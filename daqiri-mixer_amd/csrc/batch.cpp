@@ -187,15 +187,17 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
         MIXER_HIP(hipMemset(J.G, 0, gbytes));
         MIXER_TRY(balloc(b, (void**)&J.img, ((size_t)c.n_chan * c.ng * c.ng + 2 * c.n_chan) * sizeof(float)));
         MIXER_TRY(get_twiddles(c.ng, &J.tw));
+        XcorrPlan p;
+        MIXER_TRY(get_plan(c, &p));
         if (nb > 8) {
-            XcorrPlan p;
-            MIXER_TRY(get_plan(c, &p));
             J.sblocks = (const int2*)p.d_tasks;
             J.n_sb = p.n_tasks;
         }
+        J.segs = (const int4*)p.d_segs;
+        J.n_segs = p.n_segs;
         J.ct_wg0 = ct;  // corner-turn workgroups only for jobs correlated from tiles
         if (!J.fused) ct += n_tb * nb * ((c.n_chan + 7) / 8);
-        J.int_wg0 = in; in += nb * c.n_chan;
+        J.int_wg0 = in; in += p.n_segs * c.n_chan;
         J.img_wg0 = im; im += c.n_chan;
         b->max_ng = std::max(b->max_ng, c.ng);
         b->payload += (uint64_t)n_tb * c.n_elem * mixer_payload_bytes(&c);

@@ -168,8 +168,9 @@ int finish_integration(mixer_ctx* x) {
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0));
     const bool image = !(x->cfg.flags & MIXER_CFG_NO_IMAGE);
-    MIXER_HIP(launch_integrate(x->s_comp, x->d_acc, x->d_v[b], x->d_vcum, image ? x->d_G : nullptr,
-                               x->cfg.n_elem, x->cfg.n_chan, x->n_slots, x->cfg.grid_x, x->cfg.ng));
+    MIXER_HIP(launch_integrate(x->s_comp, x->plan, x->d_acc, x->d_v[b], x->d_vcum,
+                               image ? x->d_G : nullptr, x->cfg.n_elem, x->cfg.n_chan,
+                               x->cfg.grid_x, x->cfg.ng));
     MIXER_TRY(time_end(x, 2, t0));
     if (image) {
         MIXER_TRY(time_begin(x, &t0));

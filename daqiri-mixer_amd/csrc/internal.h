@@ -15,6 +15,7 @@ int plan_slots(const mixer_array_cfg& c);
 // contexts correlate straight from the heaps (no corner-turn pass) under the tuning in effect
 bool plan_fused(const mixer_array_cfg& c);
 std::vector<std::pair<int, int>> sblocks_for(int nb);
+std::vector<int4> segs_for(int nb);
 int get_plan(const mixer_array_cfg& c, XcorrPlan* p);
 int get_twiddles(int ng, const float2** out);
 int current_device();

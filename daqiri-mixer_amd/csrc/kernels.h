@@ -21,6 +21,8 @@ struct XcorrPlan {
     int fused;            // contexts: 1 = correlate straight from the heaps, 0 = via tiles
     int n_tasks;          // BLOCKED only
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
+    int n_segs;           // integrate work items
+    const void* d_segs;   // int4 (d, j0, j1, 0): tiles (J + d, J), j0 <= J < j1
 };
 
 // Where the correlator reads its 32-sample element rows. A row (channel c, time block tb,
@@ -51,8 +53,10 @@ hipError_t launch_corner_turn(hipStream_t s, const uint8_t* heaps, int8_t* tiles
 // wrap (optional): counts workgroups that staged an imaginary -128
 hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& plan, const XSrc& src, int32_t* acc,
                         uint32_t* wrap, int C, int n_tb, int accumulate);
-hipError_t launch_integrate(hipStream_t s, const int32_t* acc, float* v_store, float* v_add,
-                            unsigned long long* G, int N, int C, int n_slots, int grid_x, int ng);
+// integrate (+ grid into G when non-null) the plan's accumulator slots, one workgroup per
+// tile-diagonal segment of the plan and channel
+hipError_t launch_integrate(hipStream_t s, const XcorrPlan& p, const int32_t* acc, float* v_store,
+                            float* v_add, unsigned long long* G, int N, int C, int grid_x, int ng);
 size_t image_lds_bytes(int ng);
 hipError_t set_image_lds_limit(int ng);
 hipError_t launch_image(hipStream_t s, unsigned long long* G, const float2* tw, float* img,
@@ -72,6 +76,8 @@ struct BatchJobDev {
     float* img;                  // [n_chan][ng][ng], then vmax[n_chan], vmin[n_chan]
     const float2* tw;            // FFT twiddles for ng
     const int2* sblocks;         // BLOCKED super-blocks (I0, J0)
+    const int4* segs;            // integrate tile-diagonal segments
+    int n_segs;
     uint32_t* wrap;              // corner-turn -128 counter
     int n_elem, n_chan, n_tb, n_slots, grid_x, ng, nb, n_sb;
     int ct_wg0, int_wg0, img_wg0;  // first workgroup of this job in each batched launch

@@ -33,9 +33,9 @@ __global__ __launch_bounds__(256) void k_integrate(const int32_t* __restrict__ a
                                                    float* __restrict__ v_store,
                                                    float* __restrict__ v_add,
                                                    unsigned long long* __restrict__ G, int N,
-                                                   int C, int n_slots, int grid_x, int ng) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    integrate_body(acc, v_store, v_add, G, N, C, n_slots, grid_x, ng, blockIdx.x, blockIdx.y, smem);
+                                                   int C, int n_slots, int grid_x, int ng,
+                                                   const int4* __restrict__ segs) {
+    integrate_seg(acc, v_store, v_add, G, N, C, n_slots, grid_x, ng, segs[blockIdx.x], blockIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_image(unsigned long long* __restrict__ G,
@@ -125,11 +125,10 @@ hipError_t launch_corner_turn(hipStream_t s, const uint8_t* heaps, int8_t* tiles
     return hipGetLastError();
 }
 
-hipError_t launch_integrate(hipStream_t s, const int32_t* acc, float* v_store, float* v_add,
-                            unsigned long long* G, int N, int C, int n_slots, int grid_x, int ng) {
-    const size_t lds = G ? (size_t)ng * ng * 16 : 0;
-    hipLaunchKernelGGL(k_integrate, dim3(N / 32, C), dim3(256), lds, s, acc, v_store, v_add, G, N,
-                       C, n_slots, grid_x, ng);
+hipError_t launch_integrate(hipStream_t s, const XcorrPlan& p, const int32_t* acc, float* v_store,
+                            float* v_add, unsigned long long* G, int N, int C, int grid_x, int ng) {
+    hipLaunchKernelGGL(k_integrate, dim3(p.n_segs, C), dim3(256), 0, s, acc, v_store, v_add, G, N,
+                       C, p.n_slots, grid_x, ng, (const int4*)p.d_segs);
     return hipGetLastError();
 }
 
@@ -143,12 +142,8 @@ hipError_t launch_image(hipStream_t s, unsigned long long* G, const float2* tw, 
 }
 
 hipError_t set_image_lds_limit(int ng) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_image,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)image_lds_bytes(ng));
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_integrate, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)((size_t)ng * ng * 16));
+    return hipFuncSetAttribute((const void*)k_image, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)image_lds_bytes(ng));
 }
 
 hipError_t launch_xcorr_plain(hipStream_t s, const int8_t* tiles, int32_t* vre, int32_t* vim, int N,

@@ -257,13 +257,12 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
 
 __global__ __launch_bounds__(256) void k_batch_integrate(const BatchJobDev* __restrict__ jobs,
                                                          int n_jobs) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int b = blockIdx.x;
     const int j = job_of(jobs, n_jobs, b, [](const BatchJobDev& J) { return J.int_wg0; });
     const BatchJobDev& J = jobs[j];
     const int local = b - J.int_wg0;
-    integrate_body(J.acc, J.v, nullptr, J.G, J.n_elem, J.n_chan, J.n_slots, J.grid_x, J.ng,
-                   local % J.nb, local / J.nb, smem);
+    integrate_seg(J.acc, J.v, nullptr, J.G, J.n_elem, J.n_chan, J.n_slots, J.grid_x, J.ng,
+                  J.segs[local % J.n_segs], local / J.n_segs);
 }
 
 __global__ __launch_bounds__(256) void k_batch_image(const BatchJobDev* __restrict__ jobs,
@@ -294,8 +293,7 @@ hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4
 
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
                                   int max_ng) {
-    hipLaunchKernelGGL(k_batch_integrate, dim3(n_wgs), dim3(256), (size_t)max_ng * max_ng * 16, s,
-                       jobs, n_jobs);
+    hipLaunchKernelGGL(k_batch_integrate, dim3(n_wgs), dim3(256), 0, s, jobs, n_jobs);
     return hipGetLastError();
 }
 
@@ -307,12 +305,8 @@ hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs
 }
 
 hipError_t set_batch_lds_limits(int max_ng) {
-    const int bytes = max_ng * max_ng * 16;
-    hipError_t e = hipFuncSetAttribute((const void*)k_batch_integrate,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_batch_image, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               bytes);
+                               max_ng * max_ng * 16);
 }
 
 }  // namespace mixer

@@ -64,129 +64,124 @@ __device__ __forceinline__ void corner_turn_body(const uint8_t* __restrict__ hea
 }
 
 // ------------------------------------------------------------------------------------------
-// Gridding of one 32x32 accumulator tile (I, J), I >= J, held in LDS as T[plane][i][j].
-// grid_x divides 32, so a grid_x x grid_x sub-block of the tile spans one grid row of elements
-// on each side and every diagonal of it is one baseline cell (dy, dx) = ((qa-qb), (pa-pb)) mod
-// ng: the diagonal is summed first, then added with one int64 LDS atomic (and once more,
-// conjugated, at (-dy, -dx) for tiles below the diagonal, whose mirror is the upper triangle).
+// Integrate + grid one segment of a tile diagonal: tiles (J + d, J), J in [j0, j1), of channel
+// c (grid: x = segment, y = channel). For each tile the accumulator slots are summed in int32
+// (exact) and converted once to f32; the tile is written as V block (I, J) and, for d > 0, as
+// the conjugate-transposed block (J, I) through an LDS transpose, so every accumulator tile is
+// read once (v_store overwrites, v_add accumulates; either may be null).
+// Gridding (G non-null): grid_x (gs) divides 32, so a gs x gs sub-block (ri, rj) of tile (I, J)
+// pairs one grid row of elements with one other and each of its diagonals dd is one baseline
+// cell dy = R d + ri - rj (R = 32 / gs), dx = dd. All tiles of a segment share d, so they touch
+// the same (2R - 1)(2gs - 1) <= 105 cells: the diagonals are summed in chunks of <= 8 elements
+// (all 256 threads busy) into an int64 LDS grid of those cells, which is added once to G[c]
+// (int64 [ng*ng][2], exact) at the end, conjugated at (-dy, -dx) as well for d > 0 (the mirror
+// of a tile below the diagonal is the upper triangle). With both V pointers null it only grids.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void grid_tile(const int (*T)[32][33], unsigned long long* G, int I,
-                                          int J, int gs, int ng) {
-    const int nsub = 32 / gs, per = 2 * gs - 1, nseg = nsub * nsub * per;
-    // diagonals are summed in chunks of at most 8 elements (4 per diagonal at grid_x 32), so
-    // all 256 threads share the work; each chunk adds its partial sum to the cell
+constexpr int kSegCells = 128;  // >= max over gs of (2 * 32 / gs - 1) * (2 * gs - 1) = 105
+
+__device__ __forceinline__ void grid_seg_tile(const int (*T)[32][33], unsigned long long* Gs, int gs) {
+    const int R = 32 / gs, W = 2 * gs - 1, nseg = R * R * W;
     const int len = gs > 8 ? 8 : gs, nck = gs / len;
     for (int w = threadIdx.x; w < nseg * nck; w += 256) {
         const int sg = w / nck, k = w % nck;
-        const int sub = sg / per, d = sg % per - (gs - 1);
-        const int ri = sub / nsub, rj = sub % nsub;
-        const int i0 = d > 0 ? d : 0, i1 = d > 0 ? gs : gs + d;
+        const int sub = sg / W, dd = sg % W - (gs - 1);
+        const int ri = sub / R, rj = sub % R;
+        const int i0 = dd > 0 ? dd : 0, i1 = dd > 0 ? gs : gs + dd;
         const int c0 = i0 + k * len, c1 = c0 + len < i1 ? c0 + len : i1;
         if (c0 >= c1) continue;
         long long sr = 0, si = 0;
         for (int ip = c0; ip < c1; ++ip) {
-            sr += T[0][ri * gs + ip][rj * gs + ip - d];
-            si += T[1][ri * gs + ip][rj * gs + ip - d];
+            sr += T[0][ri * gs + ip][rj * gs + ip - dd];
+            si += T[1][ri * gs + ip][rj * gs + ip - dd];
         }
-        const int a = I * 32 + ri * gs + i0, b = J * 32 + rj * gs + i0 - d;
-        int dy = a / gs - b / gs, dx = a % gs - b % gs;
-        dy += (dy < 0) ? ng : 0;
-        dx += (dx < 0) ? ng : 0;
-        const int cell = dy * ng + dx;
-        atomicAdd(&G[2 * cell], (unsigned long long)sr);
-        atomicAdd(&G[2 * cell + 1], (unsigned long long)si);
-        if (I != J) {
-            const int cell2 = (dy ? ng - dy : 0) * ng + (dx ? ng - dx : 0);
-            atomicAdd(&G[2 * cell2], (unsigned long long)sr);
-            atomicAdd(&G[2 * cell2 + 1], (unsigned long long)(-si));
-        }
+        const int cell = (ri - rj + R - 1) * W + (dd + gs - 1);
+        atomicAdd(&Gs[2 * cell], (unsigned long long)sr);
+        atomicAdd(&Gs[2 * cell + 1], (unsigned long long)si);
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// Integrate: grid x = row block I, y = channel. Sums the accumulator slots in int32 (exact),
-// converts once to f32 and writes rows [32I, 32I+32) of both planes of the full square
-// (v_store overwrites, v_add accumulates; either may be null). Tiles above the diagonal come
-// from tile (J, I) transposed and conjugated through LDS. With G non-null it also grids the
-// lower-triangle tiles it reads into an LDS partial image grid and adds that to G[c]
-// (int64 [ng*ng][2], exact) with one global atomic per non-zero cell; with both V pointers
-// null it only grids.
-// ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void integrate_body(const int32_t* __restrict__ acc,
-                                               float* __restrict__ v_store,
-                                               float* __restrict__ v_add,
-                                               unsigned long long* __restrict__ G, int N, int C,
-                                               int n_slots, int grid_x, int ng, int I, int c,
-                                               unsigned char* smem) {
-    unsigned long long* Gp = (unsigned long long*)smem;  // [ng*ng][2] when G is non-null
+__device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
+                                              float* __restrict__ v_store,
+                                              float* __restrict__ v_add,
+                                              unsigned long long* __restrict__ G, int N, int C,
+                                              int n_slots, int grid_x, int ng, int4 seg, int c) {
     __shared__ int T[2][32][33];
+    __shared__ unsigned long long Gs[2 * kSegCells];
+    const int d = seg.x, j0 = seg.y, j1 = seg.z;
     const int nb = N / 32, NT = nb * (nb + 1) / 2;
-    const int tid = threadIdx.x, i = tid >> 3, j0 = (tid & 7) * 4;
+    const int tid = threadIdx.x, i = tid >> 3, jq = (tid & 7) * 4;
     const size_t slot_stride = (size_t)C * NT * 2048;
     const int32_t* ac = acc + (size_t)c * NT * 2048;
     const bool write_v = v_store || v_add;
-    const int nn = ng * ng;
-    if (G) {
-        for (int q = tid; q < 2 * nn; q += 256) Gp[q] = 0ull;
-    }
-    const int jend = write_v ? nb : I + 1;
-    // the slot-summed tile of column block J, this thread's 2 x 4 values
+    const int gs = grid_x, R = 32 / gs, W = 2 * gs - 1, ncell = (2 * R - 1) * W;
+    if (G) Gs[tid] = 0ull;  // 256 = 2 * kSegCells entries
+    // the slot-summed tile (J + d, J): this thread's row i, columns jq..jq+3, both planes
     auto load = [&](int J, v4i* x) {
-        const bool lower = J <= I;
-        const int32_t* t = ac + (size_t)(lower ? I * (I + 1) / 2 + J : J * (J + 1) / 2 + I) * 2048 +
-                           i * 32 + j0;
+        const int I = J + d;
+        const int32_t* t = ac + ((size_t)I * (I + 1) / 2 + J) * 2048 + i * 32 + jq;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             x[p] = *(const v4i*)(t + p * 1024);
             for (int sl = 1; sl < n_slots; ++sl) x[p] += *(const v4i*)(t + sl * slot_stride + p * 1024);
         }
     };
+    auto put = [&](int row, int col, int p, float4 f) {  // V element (row, col..col+3) of plane p
+        const size_t off = ((size_t)c * 2 + p) * N * N + (size_t)row * N + col;
+        if (v_store) *(float4*)(v_store + off) = f;
+        if (v_add) {
+            float4 o = *(float4*)(v_add + off);
+            o.x += f.x; o.y += f.y; o.z += f.z; o.w += f.w;
+            *(float4*)(v_add + off) = o;
+        }
+    };
     v4i nxt[2];
-    load(0, nxt);
-    for (int J = 0; J < jend; ++J) {
-        int vals[2][4];
-        const bool lower = J <= I;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            vals[p][0] = nxt[p].x; vals[p][1] = nxt[p].y; vals[p][2] = nxt[p].z; vals[p][3] = nxt[p].w;
-        }
-        if (J + 1 < jend) load(J + 1, nxt);  // in flight while this tile is gridded and stored
-        if (!lower || G) {
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) T[p][i][j0 + q] = vals[p][q];
-            __syncthreads();
-            if (lower) {
-                grid_tile(T, Gp, I, J, grid_x, ng);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    vals[0][q] = T[0][j0 + q][i];
-                    vals[1][q] = -T[1][j0 + q][i];
-                }
-            }
-            __syncthreads();
-        }
+    load(j0, nxt);
+    for (int J = j0; J < j1; ++J) {
+        const int I = J + d;
+        const v4i cur[2] = {nxt[0], nxt[1]};
+        if (J + 1 < j1) load(J + 1, nxt);  // in flight while this tile is stored and gridded
         if (write_v) {
 #pragma unroll
+            for (int p = 0; p < 2; ++p)
+                put(I * 32 + i, J * 32 + jq, p,
+                    make_float4((float)cur[p].x, (float)cur[p].y, (float)cur[p].z, (float)cur[p].w));
+        }
+        if (G || (write_v && d > 0)) {
+#pragma unroll
             for (int p = 0; p < 2; ++p) {
-                float4 f = make_float4((float)vals[p][0], (float)vals[p][1], (float)vals[p][2],
-                                       (float)vals[p][3]);
-                const size_t off = ((size_t)c * 2 + p) * N * N + (size_t)(I * 32 + i) * N + J * 32 + j0;
-                if (v_store) *(float4*)(v_store + off) = f;
-                if (v_add) {
-                    float4 o = *(float4*)(v_add + off);
-                    o.x += f.x; o.y += f.y; o.z += f.z; o.w += f.w;
-                    *(float4*)(v_add + off) = o;
-                }
+                T[p][i][jq + 0] = cur[p].x;
+                T[p][i][jq + 1] = cur[p].y;
+                T[p][i][jq + 2] = cur[p].z;
+                T[p][i][jq + 3] = cur[p].w;
             }
+            __syncthreads();
+            if (G) grid_seg_tile(T, Gs, gs);
+            if (write_v && d > 0) {  // block (J, I) = conj(block (I, J))^T
+                put(J * 32 + i, I * 32 + jq, 0,
+                    make_float4((float)T[0][jq][i], (float)T[0][jq + 1][i], (float)T[0][jq + 2][i],
+                                (float)T[0][jq + 3][i]));
+                put(J * 32 + i, I * 32 + jq, 1,
+                    make_float4((float)-T[1][jq][i], (float)-T[1][jq + 1][i], (float)-T[1][jq + 2][i],
+                                (float)-T[1][jq + 3][i]));
+            }
+            __syncthreads();
         }
     }
     if (G) {
-        unsigned long long* Gc = G + (size_t)c * 2 * nn;
-        for (int q = tid; q < 2 * nn; q += 256)
-            if (Gp[q]) atomicAdd(&Gc[q], Gp[q]);
+        unsigned long long* Gc = G + (size_t)c * 2 * ng * ng;
+        for (int q = tid; q < ncell; q += 256) {
+            const unsigned long long re = Gs[2 * q], im = Gs[2 * q + 1];
+            if (!re && !im) continue;
+            const int dy = R * d + q / W - (R - 1), dx = q % W - (gs - 1);
+            const int y = dy < 0 ? dy + ng : dy, x = dx < 0 ? dx + ng : dx;
+            atomicAdd(&Gc[2 * (y * ng + x)], re);
+            atomicAdd(&Gc[2 * (y * ng + x) + 1], im);
+            if (d > 0) {
+                const int y2 = y ? ng - y : 0, x2 = x ? ng - x : 0;
+                atomicAdd(&Gc[2 * (y2 * ng + x2)], re);
+                atomicAdd(&Gc[2 * (y2 * ng + x2) + 1], (unsigned long long)(-(long long)im));
+            }
+        }
     }
 }
 

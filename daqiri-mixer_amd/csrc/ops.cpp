@@ -66,7 +66,15 @@ int plan_slots(const mixer_array_cfg& c) {
 
 bool plan_fused(const mixer_array_cfg& c) {
     const mixer_tuning t = tuning_for(c.n_elem);
-    return t.staging == 1 ? true : t.staging == 2 ? false : c.n_elem / 32 <= 8;
+    return t.staging != 2;
+}
+
+// Integrate work items: each tile diagonal d (tiles (J + d, J)) in segments of at most 8 tiles.
+std::vector<int4> segs_for(int nb) {
+    std::vector<int4> v;
+    for (int d = 0; d < nb; ++d)
+        for (int j0 = 0; j0 < nb - d; j0 += 8) v.push_back(make_int4(d, j0, std::min(j0 + 8, nb - d), 0));
+    return v;
 }
 
 // Work items (I0, J0) of the BLOCKED correlator (nb % 4 == 0) covering the lower triangle
@@ -87,6 +95,7 @@ std::vector<std::pair<int, int>> sblocks_for(int nb) {
 namespace {
 std::mutex g_cache_mtx;
 std::map<std::pair<int, int>, void*> g_sblock_cache;  // (device, nb) -> int2 list
+std::map<std::pair<int, int>, void*> g_seg_cache;     // (device, nb) -> int4 list
 std::map<std::pair<int, int>, void*> g_tw_cache;      // (device, ng) -> float2 twiddles
 std::map<std::pair<int, int>, bool> g_lds_set;        // (device, ng) -> attribute set
 }  // namespace
@@ -102,14 +111,30 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     p->n_elem = c.n_elem;
     p->n_slots = plan_slots(c);
     const mixer_tuning t = tuning_for(c.n_elem);
-    // automatic choices, from timings of whole integrations on MI355X: FOLD arrays correlate
-    // fastest straight from the heaps with loads issued two blocks ahead; BLOCKED arrays
-    // reuse each element block across many super-blocks and run faster from corner-turned tiles
+    // automatic choices, from timings of whole integrations on MI355X: every size correlates
+    // fastest straight from the heaps with loads issued two blocks ahead (256 x 128 x 8192:
+    // 0.31 ms vs 0.45 ms with a corner-turn pass; 1024 x 256 x 4096: 3.97 vs 4.12 ms)
     p->fused = plan_fused(c) ? 1 : 0;
-    p->prefetch = t.prefetch > 0 ? t.prefetch : (p->fused && nb <= 8 ? 3 : 1);
+    p->prefetch = t.prefetch > 0 ? t.prefetch : 3;
     p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
     p->n_tasks = 0;
     p->d_tasks = nullptr;
+    {
+        const auto sg = segs_for(nb);
+        p->n_segs = (int)sg.size();
+        std::lock_guard<std::mutex> lk(g_cache_mtx);
+        auto key = std::make_pair(current_device(), nb);
+        auto it = g_seg_cache.find(key);
+        if (it == g_seg_cache.end()) {
+            void* d = nullptr;
+            int st = hip_status(hipMalloc(&d, sg.size() * sizeof(int4)));
+            if (st) return st;
+            st = hip_status(hipMemcpy(d, sg.data(), sg.size() * sizeof(int4), hipMemcpyHostToDevice));
+            if (st) return st;
+            it = g_seg_cache.emplace(key, d).first;
+        }
+        p->d_segs = it->second;
+    }
     if (nb <= 8) {
         p->kind = XcorrPlan::FOLD;
         return MIXER_OK;
@@ -290,9 +315,11 @@ int mixer_integrate(void* stream, const mixer_array_cfg* c, const int32_t* d_acc
     int st = mixer_validate_cfg(c);
     if (st) return st;
     if (!d_acc || !d_v) return MIXER_ERR_ARG;
-    return hip_status(launch_integrate((hipStream_t)stream, d_acc, accumulate ? nullptr : d_v,
+    XcorrPlan p;
+    if ((st = get_plan(*c, &p))) return st;
+    return hip_status(launch_integrate((hipStream_t)stream, p, d_acc, accumulate ? nullptr : d_v,
                                        accumulate ? d_v : nullptr, nullptr, c->n_elem, c->n_chan,
-                                       plan_slots(*c), c->grid_x, c->ng));
+                                       c->grid_x, c->ng));
 }
 
 // Gridding runs as the grid-only form of the integrate kernel into a stream-ordered int64
@@ -304,14 +331,16 @@ int mixer_image(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, fl
     if (!d_acc || !d_img || !d_vmax || !d_vmin) return MIXER_ERR_ARG;
     const float2* tw = nullptr;
     if ((st = get_twiddles(c->ng, &tw))) return st;
+    XcorrPlan p;
+    if ((st = get_plan(*c, &p))) return st;
     hipStream_t s = (hipStream_t)stream;
     const size_t gbytes = (size_t)c->n_chan * c->ng * c->ng * 16;
     void* g = nullptr;
     if ((st = hip_status(hipMallocAsync(&g, gbytes, s)))) return st;
     st = hip_status(hipMemsetAsync(g, 0, gbytes, s));
     if (!st)
-        st = hip_status(launch_integrate(s, d_acc, nullptr, nullptr, (unsigned long long*)g, c->n_elem,
-                                         c->n_chan, plan_slots(*c), c->grid_x, c->ng));
+        st = hip_status(launch_integrate(s, p, d_acc, nullptr, nullptr, (unsigned long long*)g,
+                                         c->n_elem, c->n_chan, c->grid_x, c->ng));
     if (!st)
         st = hip_status(launch_image(s, (unsigned long long*)g, tw, d_img, d_vmax, d_vmin, c->n_chan,
                                      c->ng));

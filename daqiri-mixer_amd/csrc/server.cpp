@@ -212,8 +212,8 @@ int finish_integration(mixer_server* S, SrvArray* a, uint64_t arrival) {
     const mixer_array_cfg& c = a->cfg;
     const int b = (int)(a->n_integ % 2);
     MIXER_HIP(hipStreamWaitEvent(a->s, a->ev_pub[b], 0));
-    MIXER_HIP(launch_integrate(a->s, a->d_acc, a->d_v[b], nullptr, a->d_G, c.n_elem, c.n_chan,
-                               a->plan.n_slots, c.grid_x, c.ng));
+    MIXER_HIP(launch_integrate(a->s, a->plan, a->d_acc, a->d_v[b], nullptr, a->d_G, c.n_elem,
+                               c.n_chan, c.grid_x, c.ng));
     float* img = a->d_img[b];
     MIXER_HIP(launch_image(a->s, a->d_G, a->tw, img, img + a->img_floats, img + a->img_floats + c.n_chan,
                            c.n_chan, c.ng));

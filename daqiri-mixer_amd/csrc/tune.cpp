@@ -45,8 +45,7 @@ int run_once(const mixer_array_cfg& c, Bufs& b, int n_tb, bool ct_only) {
     MIXER_HIP(launch_xcorr(b.s, p, xs, b.acc, nullptr, c.n_chan, n_tb, 0));
     const float2* tw = nullptr;
     MIXER_TRY(get_twiddles(c.ng, &tw));
-    MIXER_HIP(launch_integrate(b.s, b.acc, b.v, nullptr, b.G, c.n_elem, c.n_chan, p.n_slots,
-                               c.grid_x, c.ng));
+    MIXER_HIP(launch_integrate(b.s, p, b.acc, b.v, nullptr, b.G, c.n_elem, c.n_chan, c.grid_x, c.ng));
     const size_t nimg = (size_t)c.n_chan * c.ng * c.ng;
     MIXER_HIP(launch_image(b.s, b.G, tw, b.img, b.img + nimg, b.img + nimg + c.n_chan, c.n_chan, c.ng));
     return MIXER_OK;

@@ -145,8 +145,8 @@ int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, 
  *   staging   how contexts feed the correlator: 1 = straight from the heaps (the corner-turn
  *             is fused into the correlator's loads, mixer_xcorr_heaps), 2 = a separate
  *             corner-turn pass into tiles (mixer_corner_turn + mixer_xcorr)
- * 0 in a field means automatic: staging 1 with prefetch 3 for arrays of <= 256 elements,
- * staging 2 with prefetch 1 above (the measured faster choices on MI355X). The table is process-wide. slots changes the accumulator
+ * 0 in a field means automatic: staging 1 with prefetch 3 (the measured fastest choice on
+ * MI355X for both the fold and the blocked correlator). The table is process-wide. slots changes the accumulator
  * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
  * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
  * src/cgemm.cuh:18-24). */
