@@ -1,0 +1,246 @@
+// corr_quad.h -- the QUAD correlator for 256-element arrays staged straight from the heaps.
+//
+// Staging from the heaps, a one-channel workgroup fetches 32-byte runs: every 128-byte line of
+// a heap plane holds 4 channels, so each of its reads uses a quarter of a line and the gather is
+// bound by the number of L2 requests. Here a workgroup owns 4 adjacent channels (one line per
+// element and plane) and a quarter of the 8-block triangle, so each staged request is a whole
+// line and the register budget (5 tiles per wave) is unchanged:
+//   item 0 / 1  triangle of blocks 0-3 / 4-7 (10 tiles per channel): wave (ch, s) runs the fold
+//               pair (s, 3 - s) of channel ch (5 tiles)
+//   item 2 / 3  rectangle rows 4-5 / 6-7 x columns 0-3 (8 tiles per channel): wave (ch, s) runs
+//               row s of channel ch against the 4 columns (4 tiles)
+// Per channel the four items cover the 36 tiles of the triangle exactly once.
+#pragma once
+#include "corr_impl.h"
+
+namespace mixer {
+
+template <int NE>
+struct QuadLds {
+    static constexpr int PS = NE * 32 + 128;   // plane stride: +128 B staggers the im plane's banks
+    static constexpr int CS = 2 * PS + 32;     // channel stride: +32 B staggers the channels' banks
+    static constexpr int BYTES = 4 * CS;
+};
+
+// Staging of one 32-sample block of NBLK element blocks for 4 channels: chunk u (16 B) is
+// element u / 16, plane (u / 8) % 2, channel (u / 2) % 4, half u % 2, so 8 consecutive chunks
+// are one contiguous 128-byte line and a wave's load covers 8 whole lines.
+template <int NBLK>
+struct QuadStager {
+    static constexpr int NE = NBLK * 32;
+    static constexpr int CHUNKS = NE * 16;
+    static constexpr int PER = CHUNKS / 512;
+    v4i r[PER];
+    template <typename GB>
+    __device__ __forceinline__ void load(const int8_t* sl, const XSrc& m, GB gblk, int q4, int C) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int u = threadIdx.x + i * 512;
+            const int el = u >> 4, p = (u >> 3) & 1, ch = (u >> 1) & 3, h = u & 1;
+            const int c = q4 + ch < C ? q4 + ch : C - 1;  // ragged last quad: re-read a valid channel
+            r[i] = *(const v4i*)(sl + m.off(p, gblk(el >> 5) * 32 + (el & 31), h) + (size_t)c * m.chan);
+        }
+    }
+    __device__ __forceinline__ void store(uint8_t* L) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int u = threadIdx.x + i * 512;
+            const int el = u >> 4, p = (u >> 3) & 1, ch = (u >> 1) & 3, h = u & 1;
+            const int unit = 2 * el + (h ^ (((el >> 2) ^ (el >> 3)) & 1));
+            *(v4i*)(L + ch * QuadLds<NE>::CS + p * QuadLds<NE>::PS + unit * 16) = r[i];
+        }
+    }
+    __device__ __forceinline__ unsigned saw80() const {
+        unsigned s = 0;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int u = threadIdx.x + i * 512;
+            if ((u >> 3) & 1) s |= has80(r[i].x) | has80(r[i].y) | has80(r[i].z) | has80(r[i].w);
+        }
+        return s;
+    }
+};
+
+// Fragment of LDS block slot B of one channel region (the per-channel layout of lds_unit with
+// the QuadLds plane stride).
+template <int NE>
+__device__ __forceinline__ Frag quad_frag(const uint8_t* Lc, int B, int lrow, int kg) {
+    const int e = B * 32 + lrow;
+    const int unit = 2 * e + (kg ^ (((e >> 2) ^ (e >> 3)) & 1));
+    Frag f;
+    f.re = *(const v4i*)(Lc + unit * 16);
+    f.im = *(const v4i*)(Lc + QuadLds<NE>::PS + unit * 16);
+    f.ng = neg8x16(f.im);
+    return f;
+}
+
+// MFMAs of one block: row slot RA against column slots [A0, A1), optional row slot RB against
+// [B0, B1) (RB < 0: none).
+template <int NE, int RA, int A0, int A1, int RB, int B0, int B1>
+__device__ __forceinline__ void quad_mfma(const uint8_t* Lc, int lrow, int kg, v16i* ar, v16i* ai,
+                                          v16i* br, v16i* bi) {
+    constexpr int JLO = (RB >= 0 && B0 < A0) ? B0 : A0;
+    constexpr int JHI = (RB >= 0 && B1 > A1) ? B1 : A1;
+    const Frag fa = quad_frag<NE>(Lc, RA, lrow, kg);
+    Frag fb = fa;
+    if constexpr (RB >= 0) fb = quad_frag<NE>(Lc, RB, lrow, kg);
+    static_for<JLO, JHI>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        constexpr bool inA = J >= A0 && J < A1;
+        constexpr bool inB = RB >= 0 && J >= B0 && J < B1;
+        if constexpr (inA || inB) {
+            Frag b;
+            if constexpr (J == RA) b = fa;
+            else if constexpr (J == RB) b = fb;
+            else b = quad_frag<NE>(Lc, J, lrow, kg);
+            if constexpr (inA) tile_step(fa, b, ar[J - A0], ai[J - A0]);
+            if constexpr (inB) tile_step(fb, b, br[J - B0], bi[J - B0]);
+        }
+    });
+}
+
+// One wave's program over the item's time range. TR maps a row/column LDS slot to the global
+// element block; the tiles are (TR(RA), TR(J)) for J in [A0, A1) and (TR(RB), TR(J)) for J in
+// [B0, B1). V = 1: loads one block ahead; V = 3: two blocks ahead (two register sets).
+template <int V, int NBLK, int RA, int A0, int A1, int RB, int B0, int B1, typename GB>
+__device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const XSrc& m, GB gblk,
+                                          int q4, int C, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                          int ch, bool valid, int32_t* __restrict__ acc,
+                                          int accumulate, bool chk, unsigned& saw) {
+    constexpr int NE = NBLK * 32;
+    constexpr int NA = A1 - A0;
+    constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
+    const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
+    v16i ar[NA], ai[NA], br[NBT], bi[NBT];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) { ar[j] = (v16i){0}; ai[j] = (v16i){0}; }
+#pragma unroll
+    for (int j = 0; j < NBT; ++j) { br[j] = (v16i){0}; bi[j] = (v16i){0}; }
+    const int coff = ch * QuadLds<NE>::CS;
+    using St = QuadStager<NBLK>;
+    auto compute = [&](const uint8_t* L) {
+        quad_mfma<NE, RA, A0, A1, RB, B0, B1>(L + coff, lrow, kg, ar, ai, br, bi);
+    };
+    if constexpr (V == 1) {
+        St st;
+        st.load(src, m, gblk, q4, C);
+        if (chk) saw |= st.saw80();
+        st.store(lds0);
+        __syncthreads();
+        for (int kb = 0; kb < n_kb; ++kb) {
+            const bool more = kb + 1 < n_kb;
+            if (more) st.load(src + (size_t)(kb + 1) * m.tb, m, gblk, q4, C);
+            compute((kb & 1) ? lds1 : lds0);
+            if (more) {
+                if (chk) saw |= st.saw80();
+                st.store((kb & 1) ? lds0 : lds1);
+            }
+            __syncthreads();
+        }
+    } else {
+        // block j in register set j & 1, both parities written out so the sets keep fixed
+        // registers (the wait before each store covers only the older set's loads)
+        St s0, s1;
+        s0.load(src, m, gblk, q4, C);
+        s1.load(src + (size_t)(n_kb > 1 ? 1 : 0) * m.tb, m, gblk, q4, C);
+        if (chk) saw |= s0.saw80();
+        s0.store(lds0);
+        __syncthreads();
+        const int8_t* nx = src + 2 * m.tb;
+        int kb = 0;
+        for (; kb + 2 <= n_kb; kb += 2) {
+            if (kb + 2 < n_kb) s0.load(nx, m, gblk, q4, C);
+            compute(lds0);
+            if (chk) saw |= s1.saw80();
+            s1.store(lds1);
+            __syncthreads();
+            if (kb + 3 < n_kb) s1.load(nx + m.tb, m, gblk, q4, C);
+            compute(lds1);
+            if (kb + 2 < n_kb) {
+                if (chk) saw |= s0.saw80();
+                s0.store(lds0);
+            }
+            __syncthreads();
+            nx += 2 * m.tb;
+        }
+        if (kb < n_kb) {  // odd tail: block kb in lds0
+            compute(lds0);
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+    const int lr = lrow;
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+        const int I = gblk(RA), J = gblk(A0 + j);
+        int32_t* t = acc + ((size_t)I * (I + 1) / 2 + J) * 2048;
+        store_tile(ar[j], t, lr, kg, accumulate);
+        store_tile(ai[j], t + 1024, lr, kg, accumulate);
+    }
+    if constexpr (RB >= 0) {
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+            const int I = gblk(RB), J = gblk(B0 + j);
+            int32_t* t = acc + ((size_t)I * (I + 1) / 2 + J) * 2048;
+            store_tile(br[j], t, lr, kg, accumulate);
+            store_tile(bi[j], t + 1024, lr, kg, accumulate);
+        }
+    }
+}
+
+// grid: 4 items x ceil(C / 4) quads x n_slots (XCD-mapped: consecutive quads of a slot on one
+// XCD, the 4 items of a quad adjacent so they share its lines in L2), 512 threads.
+template <int V>
+__global__ __launch_bounds__(512)
+void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap, int C,
+                  int n_tb, int n_slots, int accumulate) {
+    constexpr int NT = 36;
+    __shared__ __attribute__((aligned(16))) uint8_t lds0[QuadLds<192>::BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds1[QuadLds<192>::BYTES];
+    const int nq = (C + 3) / 4, total = 4 * nq * n_slots;
+    const int logical = xcd_logical(blockIdx.x, total);
+    if (logical >= total) return;  // uniform over the workgroup
+    const int item = logical & 3, q = (logical >> 2) % nq, slot = (logical >> 2) / nq;
+    const int kb0 = (int)((long)n_tb * slot / n_slots);
+    const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
+    if (kb1 <= kb0) return;
+    const int w = threadIdx.x >> 6, ch = w >> 1, s = w & 1;
+    const int q4 = 4 * q, c = q4 + ch;
+    const bool valid = c < C;
+    int32_t* a = acc + ((size_t)slot * C + (valid ? c : 0)) * NT * 2048;
+    const int8_t* src = xs.base + (size_t)kb0 * xs.tb;
+    const int n_kb = kb1 - kb0;
+    const bool chk = wrap != nullptr;
+    unsigned saw = 0;
+    if (item < 2) {
+        const int d0 = 4 * item;
+        auto gblk = [=](int b) { return d0 + b; };
+        if (s == 0)
+            quad_wave<V, 4, 3, 0, 4, 0, 0, 1>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                              accumulate, chk, saw);
+        else
+            quad_wave<V, 4, 2, 0, 3, 1, 0, 2>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                              accumulate, chk, saw);
+    } else {
+        const int r0 = item == 2 ? 4 : 6;  // LDS slots 0, 1: rows r0, r0 + 1; slots 2..5: columns 0..3
+        auto gblk = [=](int b) { return b < 2 ? r0 + b : b - 2; };
+        if (s == 0)
+            quad_wave<V, 6, 0, 2, 6, -1, 0, 0>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                               accumulate, chk, saw);
+        else
+            quad_wave<V, 6, 1, 2, 6, -1, 0, 0>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                               accumulate, chk, saw);
+    }
+    count_wrap(wrap, saw);
+}
+
+template <int V>
+inline hipError_t launch_quad_t(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                                uint32_t* wrap, int C, int n_tb, int accumulate) {
+    const int wgs = 4 * ((C + 3) / 4) * p.n_slots;
+    hipLaunchKernelGGL((k_xcorr_quad<V>), dim3((wgs + 7) / 8 * 8), dim3(512), 0, s, src, acc, wrap,
+                       C, n_tb, p.n_slots, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace mixer

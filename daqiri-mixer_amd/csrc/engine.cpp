@@ -92,7 +92,7 @@ struct mixer_ctx {
     std::vector<hipEvent_t> ev_pool;
 
     // timing
-    bool timing = false;
+    unsigned timing = 0;  // bit s: time stage s
     std::vector<TimedLaunch> timed;
     double t_ms[4] = {0, 0, 0, 0};
     uint64_t t_n[4] = {0, 0, 0, 0};
@@ -131,13 +131,13 @@ int resolve_timing(mixer_ctx* x, bool wait) {
     return MIXER_OK;
 }
 
-int time_begin(mixer_ctx* x, hipEvent_t* a) {
-    if (!x->timing) return MIXER_OK;
+int time_begin(mixer_ctx* x, hipEvent_t* a, int stage) {
+    if (!(x->timing & (1u << stage))) return MIXER_OK;
     MIXER_HIP(hipEventCreate(a));
     return hip_status(hipEventRecord(*a, x->s_comp));
 }
 int time_end(mixer_ctx* x, int stage, hipEvent_t a) {
-    if (!x->timing) return MIXER_OK;
+    if (!(x->timing & (1u << stage))) return MIXER_OK;
     hipEvent_t b;
     MIXER_HIP(hipEventCreate(&b));
     MIXER_HIP(hipEventRecord(b, x->s_comp));
@@ -166,14 +166,14 @@ int finish_integration(mixer_ctx* x) {
     MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pub_done[b], 0));
     if (x->d_vcum) MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pub_done_cum, 0));
     hipEvent_t t0 = nullptr;
-    MIXER_TRY(time_begin(x, &t0));
+    MIXER_TRY(time_begin(x, &t0, 2));
     const bool image = !(x->cfg.flags & MIXER_CFG_NO_IMAGE);
     MIXER_HIP(launch_integrate(x->s_comp, x->plan, x->d_acc, x->d_v[b], x->d_vcum,
                                image ? x->d_G : nullptr, x->cfg.n_elem, x->cfg.n_chan,
                                x->cfg.grid_x, x->cfg.ng));
     MIXER_TRY(time_end(x, 2, t0));
     if (image) {
-        MIXER_TRY(time_begin(x, &t0));
+        MIXER_TRY(time_begin(x, &t0, 3));
         float* img = x->d_img[b];
         MIXER_HIP(launch_image(x->s_comp, x->d_G, x->d_tw, img, img + x->img_floats,
                                img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng));
@@ -195,7 +195,7 @@ int finish_integration(mixer_ctx* x) {
 int run_update(mixer_ctx* x, const uint8_t* src) {
     if (x->plan.fused) return MIXER_OK;
     hipEvent_t t0 = nullptr;
-    MIXER_TRY(time_begin(x, &t0));
+    MIXER_TRY(time_begin(x, &t0, 0));
     MIXER_HIP(launch_corner_turn(x->s_comp, src, x->d_tiles, x->d_wrap, x->cfg.n_elem,
                                  x->cfg.n_chan, x->n_tb_upd, 0, x->cfg.n_chan, x->plan.ct_ch));
     MIXER_TRY(time_end(x, 0, t0));
@@ -204,7 +204,7 @@ int run_update(mixer_ctx* x, const uint8_t* src) {
 // Correlate one V-update (from the heaps at src when fused, else from the tiles).
 int run_gemm(mixer_ctx* x, const uint8_t* src) {
     hipEvent_t t0 = nullptr;
-    MIXER_TRY(time_begin(x, &t0));
+    MIXER_TRY(time_begin(x, &t0, 1));
     const XSrc xs = x->plan.fused ? xsrc_heaps(src, x->cfg.n_elem, x->cfg.n_chan)
                                   : xsrc_tiles(x->d_tiles, x->cfg.n_elem, x->n_tb_upd);
     MIXER_HIP(launch_xcorr(x->s_comp, x->plan, xs, x->d_acc, x->plan.fused ? x->d_wrap : nullptr,
@@ -403,8 +403,10 @@ int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arriv
     MIXER_HIP(hipSetDevice(x->device));
     x->last_arrival_ns = arrival_ns;
     int done = 0;
+    bool copied = false;  // a copy of the caller's buffer is queued on s_copy
     // complete a partly filled ring buffer first
     while (done < n_tb && x->ring_fill > 0) {
+        copied = true;
         const int take = std::min(n_tb - done, x->n_tb_upd - x->ring_fill);
         MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur] + (size_t)x->ring_fill * x->tb_bytes,
                                  d + (size_t)done * x->tb_bytes, (size_t)take * x->tb_bytes,
@@ -426,14 +428,18 @@ int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arriv
     // leftovers into the ring
     if (done < n_tb) {
         const int take = n_tb - done;
+        copied = true;
         MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur], d + (size_t)done * x->tb_bytes,
                                  (size_t)take * x->tb_bytes, hipMemcpyDeviceToDevice, x->s_copy));
         x->ring_fill = take;
         x->st.heaps_in += (uint64_t)take * x->cfg.n_elem;
     }
-    // the caller's buffer is free once both streams are past it
-    MIXER_HIP(hipEventRecord(x->copy_done, x->s_copy));
-    MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
+    // the caller's buffer is free once both streams are past it (a cross-stream wait only when
+    // something was copied: it costs the compute stream an idle gap of tens of microseconds)
+    if (copied) {
+        MIXER_HIP(hipEventRecord(x->copy_done, x->s_copy));
+        MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
+    }
     MIXER_TRY(new_token(x, x->s_comp, token));
     { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
     return MIXER_OK;
@@ -563,7 +569,7 @@ int mixer_set_timing(mixer_ctx* x, int enable) {
     MIXER_HIP(hipSetDevice(x->device));
     MIXER_TRY(resolve_timing(x, true));
     for (int i = 0; i < 4; ++i) { x->t_ms[i] = 0; x->t_n[i] = 0; }
-    x->timing = enable != 0;
+    x->timing = (unsigned)enable & 0xFu;
     return MIXER_OK;
 }
 

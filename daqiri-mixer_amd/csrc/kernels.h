@@ -6,6 +6,9 @@
 namespace mixer {
 
 // How the correlator covers the lower triangle of one channel.
+//   QUAD     256-element arrays staged straight from the heaps (prefetch 1 or 3): one
+//            workgroup per (4 adjacent channels, quarter of the triangle) so that every staged
+//            load is a whole 128-byte line of a heap plane.
 //   FOLD     arrays of <= 8 blocks of 32 elements: one workgroup per (channel, slot), one wave
 //            per folded row pair (r, nb-1-r); balanced (nb+1 tiles per wave).
 //   BLOCKED  larger arrays: one workgroup per 4x8-tile super-block, one wave per 2x2 tile
@@ -13,7 +16,7 @@ namespace mixer {
 // n_slots splits the time axis of one call into partial int32 sums (summed exactly by the
 // integrate and image kernels) so that small channel counts still fill all 256 CUs.
 struct XcorrPlan {
-    enum Kind { FOLD = 0, BLOCKED = 1 } kind;
+    enum Kind { FOLD = 0, BLOCKED = 1, QUAD = 2 } kind;
     int n_elem;
     int n_slots;
     int prefetch;         // time-loop pipeline: 1 = registers one block ahead, 2/4 = LDS DMA stages

@@ -18,8 +18,18 @@ MIXER_DECL(blk, 3)
 MIXER_DECL(blk, 4)
 #undef MIXER_DECL
 
+hipError_t launch_quad_1(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                         uint32_t* wrap, int C, int n_tb, int accumulate);
+hipError_t launch_quad_3(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                         uint32_t* wrap, int C, int n_tb, int accumulate);
+
 hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
                         uint32_t* wrap, int C, int n_tb, int accumulate) {
+    if (p.kind == XcorrPlan::QUAD) {
+        if (p.prefetch == 3) return launch_quad_3(s, p, src, acc, wrap, C, n_tb, accumulate);
+        if (p.prefetch == 1) return launch_quad_1(s, p, src, acc, wrap, C, n_tb, accumulate);
+        return hipErrorInvalidValue;
+    }
     const bool fold = p.kind == XcorrPlan::FOLD;
     switch (p.prefetch) {
     case 1: return fold ? launch_fold_1(s, p, src, acc, wrap, C, n_tb, accumulate)

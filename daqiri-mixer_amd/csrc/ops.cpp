@@ -136,7 +136,10 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
         p->d_segs = it->second;
     }
     if (nb <= 8) {
-        p->kind = XcorrPlan::FOLD;
+        // 256-element arrays from the heaps: 4-channel QUAD items (whole-line loads); the
+        // direct-to-LDS variants keep the per-channel FOLD layout
+        const bool quad = nb == 8 && p->fused && (p->prefetch == 1 || p->prefetch == 3);
+        p->kind = quad ? XcorrPlan::QUAD : XcorrPlan::FOLD;
         return MIXER_OK;
     }
     p->kind = XcorrPlan::BLOCKED;
@@ -294,6 +297,7 @@ int mixer_xcorr(void* stream, const mixer_array_cfg* c, const int8_t* d_tiles, i
     XcorrPlan p;
     if ((st = get_plan(*c, &p))) return st;
     if (n_tb < p.n_slots) return MIXER_ERR_ARG;
+    if (p.kind == XcorrPlan::QUAD) p.kind = XcorrPlan::FOLD;  // tiles: the per-channel layout
     return hip_status(launch_xcorr((hipStream_t)stream, p, xsrc_tiles(d_tiles, c->n_elem, n_tb),
                                    d_acc, nullptr, c->n_chan, n_tb, accumulate ? 1 : 0));
 }

@@ -437,8 +437,10 @@ class Engine:
         check(lib().mixer_get_stats(self.h, C.byref(s)), "mixer_get_stats")
         return s
 
-    def set_timing(self, on: bool):
-        check(lib().mixer_set_timing(self.h, 1 if on else 0), "mixer_set_timing")
+    def set_timing(self, on, stages: int = 0xF):
+        """Time every launch of the given stages (bit mask: 1 corner-turn, 2 GEMM, 4 integrate,
+        8 image)."""
+        check(lib().mixer_set_timing(self.h, stages if on else 0), "mixer_set_timing")
 
     def timing(self, stage: int):
         ms = C.c_double(0)

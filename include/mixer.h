@@ -231,8 +231,9 @@ int mixer_publish_image(mixer_ctx* ctx, void* region, size_t region_bytes);
 
 int mixer_get_stats(mixer_ctx* ctx, mixer_stats* st);
 
-/* Kernel timing on the context's compute stream: when enabled, HIP events bracket every
- * launch of each pipeline stage. Stage ids: 0 corner-turn, 1 GEMM, 2 integrate, 3 image. */
+/* Kernel timing on the context's compute stream: HIP events bracket every launch of the
+ * stages in `enable`, a bit mask (bit s = stage s; 0 = off, 0xF = all). Stage ids: 0
+ * corner-turn, 1 GEMM, 2 integrate, 3 image. */
 int mixer_set_timing(mixer_ctx* ctx, int enable);
 int mixer_get_timing(mixer_ctx* ctx, int stage, double* total_ms, uint64_t* launches);
 void* mixer_compute_stream(mixer_ctx* ctx);
