@@ -4,6 +4,10 @@
 //   mixer_corr [cfg|-] [--seconds N] [--fps F] [--device D] [--noflop] [--flush N] [--dump PATH]
 //              [--dump-mode cumulative|integ] [--elements N] [--grid-x G] [--channels C]
 //              [--k-update K] [--ng NG] [--rate GBps] [--sigma S] [--seed X] [--shm NAME]
+//              [--rho R] [--torbit S] [--refresh-hz F] [--eth-dst MAC]
+// --rho/--torbit/--refresh-hz move the synthetic planet on the orbit (l, m) = rho (cos, sin)
+// (2 pi t / torbit), re-generating one replay slot per refresh (src/tx_host.cu:84-86,116-125);
+// --eth-dst sets the wire header's destination MAC (src/tx_host.cu:87,96-98).
 //
 // The first positional argument (the reference's DAQIRI YAML) is accepted and ignored.
 // --flush N sets the samples per integration to N V-updates (the reference flushes its int32
@@ -19,6 +23,7 @@
 #include <csignal>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -52,8 +57,10 @@ int main(int argc, char** argv) {
     bool noflop = false, dump_cum = true;
     const char* dump_path = nullptr;
     const char* shm_name = "/corr32";
-    double rate = 0, sigma = 0.5;
+    double rate = 0, sigma = 0.5, rho = 0.525, torbit = 3.0, refresh_hz = 0.0;
     uint64_t seed = 0x4D495832ull;
+    mixer_wire hdr_wire;
+    mixer_default_wire(&hdr_wire);
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto nxt = [&]() -> const char* {
@@ -76,10 +83,23 @@ int main(int argc, char** argv) {
         else if (a == "--sigma") sigma = atof(nxt());
         else if (a == "--seed") seed = strtoull(nxt(), nullptr, 0);
         else if (a == "--shm") shm_name = nxt();
+        else if (a == "--rho") rho = atof(nxt());
+        else if (a == "--torbit") torbit = atof(nxt());
+        else if (a == "--refresh-hz") refresh_hz = atof(nxt());
+        else if (a == "--eth-dst") {
+            const char* s = nxt();
+            unsigned m[6];
+            if (sscanf(s, "%x:%x:%x:%x:%x:%x", &m[0], &m[1], &m[2], &m[3], &m[4], &m[5]) != 6) {
+                fprintf(stderr, "bad MAC %s\n", s);
+                return 1;
+            }
+            for (int k = 0; k < 6; ++k) hdr_wire.eth_dst[k] = (uint8_t)m[k];
+        }
         else if (a == "-h" || a == "--help") {
             fprintf(stderr, "usage: %s [cfg] [--seconds N][--fps F][--device D][--noflop][--flush N][--dump PATH]\n"
                             "  [--dump-mode cumulative|integ][--elements N][--grid-x G][--channels C][--k-update K]\n"
-                            "  [--ng NG][--rate GBps][--sigma S][--seed X][--shm NAME]\n", argv[0]);
+                            "  [--ng NG][--rate GBps][--sigma S][--seed X][--shm NAME]\n"
+                            "  [--rho R][--torbit S][--refresh-hz F][--eth-dst MAC]\n", argv[0]);
             return 1;
         } else if (a[0] == '-' && a != "-") { fprintf(stderr, "unknown flag %s\n", a.c_str()); return 1; }
     }
@@ -115,6 +135,9 @@ int main(int argc, char** argv) {
     mixer_sky sky;
     mixer_default_sky(&sky);
     sky.sigma = sigma;
+    sky.planet_l = rho;  // orbit phase 0 (src/tx_host.cu:106)
+    sky.planet_m = 0.0;
+    CK(mixer_synth_set_wire(&hdr_wire));
     CK(mixer_synth_packets(&cfg, &sky, seed, 0, RB * N, ring, pitch, 0));
     std::vector<uint64_t> slot_tok(NSLOT, 0);
     std::atomic<uint64_t> blocks{0};
@@ -126,10 +149,20 @@ int main(int argc, char** argv) {
     std::thread src([&] {
         uint64_t seq = 0;
         const uint64_t t0 = now_ns();
+        uint64_t next_refresh = t0;
         int slot = 0;
         while (!g_stop) {
             if (slot_tok[slot]) CK(mixer_token_wait(ctx, slot_tok[slot]));  // slot free again
             uint8_t* base = ring + (size_t)slot * per_slot * N * pitch;
+            if (refresh_hz > 0 && torbit > 0 && now_ns() >= next_refresh) {
+                // the planet moved: re-generate this slot at its new position (fresh noise too)
+                const double t = (now_ns() - t0) * 1e-9, w = 2.0 * M_PI / torbit;
+                mixer_sky s2 = sky;
+                s2.planet_l = rho * std::cos(w * t);
+                s2.planet_m = rho * std::sin(w * t);
+                CK(mixer_synth_packets(&cfg, &s2, seed, seq, per_slot * N, base, pitch, 0));
+                next_refresh += (uint64_t)(1e9 / refresh_hz);
+            }
             for (int k = 0; k < per_slot * N; ++k) {  // bake the current seq (src/tx_host.cu:141)
                 const uint32_t s = (uint32_t)(seq + k);
                 uint8_t* p = base + (size_t)k * pitch + cfg.seq_byte;

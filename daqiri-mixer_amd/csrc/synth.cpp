@@ -2,9 +2,11 @@
 // DAQIRI capture (out of scope) and restates the transmitter's signal model, k_sky
 // (src/tx_host.cu:41-60), plus Gaussian noise and extra point sources. Header and seq baking
 // follow src/tx_host.cu:136-143 (64-byte header, seq big-endian at byte 48, element = seq % N).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -35,17 +37,71 @@ inline int8_t q8(double v) {
     return (int8_t)(q < -127 ? -127 : (q > 127 ? 127 : q));
 }
 
-void write_header(uint8_t* p, const mixer_array_cfg& c, uint32_t seq) {
-    memset(p, 0, (size_t)c.hdr_bytes);
-    p[c.seq_byte + 0] = (uint8_t)(seq >> 24);
-    p[c.seq_byte + 1] = (uint8_t)(seq >> 16);
-    p[c.seq_byte + 2] = (uint8_t)(seq >> 8);
-    p[c.seq_byte + 3] = (uint8_t)seq;
+void put16(uint8_t* p, uint32_t v) {
+    p[0] = (uint8_t)(v >> 8);
+    p[1] = (uint8_t)v;
 }
+void put32(uint8_t* p, uint32_t v) {
+    p[0] = (uint8_t)(v >> 24);
+    p[1] = (uint8_t)(v >> 16);
+    p[2] = (uint8_t)(v >> 8);
+    p[3] = (uint8_t)v;
+}
+
+std::mutex g_wire_mtx;
+mixer_wire g_wire;
+bool g_wire_set = false;
 
 }  // namespace
 
 extern "C" {
+
+void mixer_default_wire(mixer_wire* w) {
+    if (!w) return;
+    memset(w, 0, sizeof(*w));
+    const uint8_t dst[6] = {0xa0, 0x88, 0xc2, 0x0d, 0x5e, 0x28};  // MIXER_DEF_ETH_DST, src/config.h:56
+    memcpy(w->eth_dst, dst, 6);
+    w->ip_src = (10u << 24) | 1u;  // 10.0.0.1, src/config.h:57
+    w->ip_dst = (10u << 24) | 2u;  // 10.0.0.2, src/config.h:58
+    w->udp_port = 4096;            // MIXER_UDP_PORT, src/config.h:52
+}
+
+// The standard IPv4 header checksum: one's-complement sum of the big-endian 16-bit words of
+// the 20-byte header (checksum field zero), folded and complemented.
+int mixer_build_header(const mixer_array_cfg* c, const mixer_wire* w, uint32_t seq, uint8_t* h) {
+    if (!c || !w || !h) return MIXER_ERR_ARG;
+    if (c->hdr_bytes < 42 || c->seq_byte < 42 || c->seq_byte + 4 > c->hdr_bytes) return MIXER_ERR_ARG;
+    memset(h, 0, (size_t)c->hdr_bytes);
+    const size_t wire = mixer_wire_bytes(c);
+    memcpy(h, w->eth_dst, 6);
+    memcpy(h + 6, w->eth_src, 6);
+    put16(h + 12, 0x0800);  // ETH_P_IP
+    uint8_t* ip = h + 14;
+    ip[0] = 0x45;           // version 4, IHL 5
+    put16(ip + 2, (uint32_t)std::min<size_t>(wire - 14, 65535));
+    ip[8] = 64;             // TTL
+    ip[9] = 17;             // UDP
+    put32(ip + 12, w->ip_src);
+    put32(ip + 16, w->ip_dst);
+    uint32_t sum = 0;
+    for (int i = 0; i < 20; i += 2) sum += ((uint32_t)ip[i] << 8) | ip[i + 1];
+    while (sum >> 16) sum = (sum & 0xffffu) + (sum >> 16);
+    put16(ip + 10, ~sum & 0xffffu);
+    uint8_t* udp = h + 34;
+    put16(udp, w->udp_port);
+    put16(udp + 2, w->udp_port);
+    put16(udp + 4, (uint32_t)std::min<size_t>(wire - 34, 65535));
+    put32(h + c->seq_byte, seq);
+    return MIXER_OK;
+}
+
+int mixer_synth_set_wire(const mixer_wire* w) {
+    if (!w) return MIXER_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_wire_mtx);
+    g_wire = *w;
+    g_wire_set = true;
+    return MIXER_OK;
+}
 
 void mixer_default_sky(mixer_sky* s) {
     if (!s) return;
@@ -85,13 +141,23 @@ int mixer_synth_packets(const mixer_array_cfg* cfg, const mixer_sky* sky, uint64
     for (int ch = 0; ch < C; ++ch)
         apl[ch] = sky->planet_scale * sky->star_amp * std::sqrt(2.0 * ch / (double)(sky->nch_gen - 1));
     const double sig = sky->sigma;
+    mixer_wire wire;
+    {
+        std::lock_guard<std::mutex> lk(g_wire_mtx);
+        if (g_wire_set) wire = g_wire;
+        else mixer_default_wire(&wire);
+    }
+    std::vector<uint8_t> tmpl((size_t)c.hdr_bytes);
+    st = mixer_build_header(&c, &wire, 0, tmpl.data());
+    if (st) return st;
     auto work = [&](int i0, int i1) {
         for (int i = i0; i < i1; ++i) {
             const uint64_t seq = seq0 + (uint64_t)i;
             const int a = (int)(seq % (uint64_t)N);
             const uint64_t tb = seq / (uint64_t)N;
             uint8_t* p = out + (size_t)i * pitch;
-            write_header(p, c, (uint32_t)seq);
+            memcpy(p, tmpl.data(), (size_t)c.hdr_bytes);
+            put32(p + c.seq_byte, (uint32_t)seq);
             int8_t* re = (int8_t*)(p + c.hdr_bytes);
             int8_t* im = re + (size_t)C * T;
             for (int ch = 0; ch < C; ++ch) {

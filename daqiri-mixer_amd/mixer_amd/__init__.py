@@ -90,6 +90,11 @@ class ServeReport(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Wire(C.Structure):
+    _fields_ = [("eth_dst", C.c_uint8 * 6), ("eth_src", C.c_uint8 * 6), ("ip_src", C.c_uint32),
+                ("ip_dst", C.c_uint32), ("udp_port", C.c_uint16), ("reserved", C.c_uint16)]
+
+
 class Sky(C.Structure):
     _fields_ = [("planet_l", C.c_double), ("planet_m", C.c_double), ("sigma", C.c_double),
                 ("star_amp", C.c_double), ("planet_scale", C.c_double), ("nch_gen", C.c_int32),
@@ -180,6 +185,9 @@ SIGNATURES = [
     ("mixer_event_elapsed_ms", I32, [P, P, P]),
     ("mixer_device_sync", I32, []),
     ("mixer_default_sky", None, [P]),
+    ("mixer_default_wire", None, [P]),
+    ("mixer_build_header", I32, [P, P, C.c_uint32, P]),
+    ("mixer_synth_set_wire", I32, [P]),
     ("mixer_synth_packets", I32, [P, P, U64, U64, I32, P, SZ, I32]),
 ]
 
@@ -229,6 +237,20 @@ def default_sky(**kw) -> Sky:
     for k, v in kw.items():
         setattr(s, k, v)
     return s
+
+
+def default_wire() -> Wire:
+    w = Wire()
+    lib().mixer_default_wire(C.byref(w))
+    return w
+
+
+def build_header(cfg, seq: int, wire: Wire | None = None) -> np.ndarray:
+    """The transmitter's wire header (eth/ip/udp, seq big-endian at seq_byte)."""
+    h = np.zeros(cfg.hdr_bytes, np.uint8)
+    w = wire if wire is not None else default_wire()
+    check(lib().mixer_build_header(C.byref(cfg), C.byref(w), seq, ptr(h)), "mixer_build_header")
+    return h
 
 
 def device_count() -> int:

@@ -400,8 +400,29 @@ typedef struct mixer_sky {
 } mixer_sky;
 
 void mixer_default_sky(mixer_sky* sky);
+/* Synthetic packets seq0 .. seq0+npkts-1 at `pitch`: the wire header (mixer_build_header with
+ * the template set by mixer_synth_set_wire, default mixer_default_wire) and the k_sky payload
+ * (src/tx_host.cu:41-60,136-143) with seeded Gaussian noise. Host only. */
 int mixer_synth_packets(const mixer_array_cfg* cfg, const mixer_sky* sky, uint64_t seed,
                         uint64_t seq0, int npkts, uint8_t* out, size_t pitch, int nthreads);
+
+/* Wire header of the reference transmitter (src/tx_host.cu:62-79, src/config.h:10-14,53-58):
+ * eth (dst MAC, src MAC, ethertype IPv4), IPv4 (version 4, IHL 5, TTL 64, protocol UDP, total
+ * length = wire bytes - 14, header checksum), UDP (source = destination port, length = wire
+ * bytes - 34, checksum 0), zeros to hdr_bytes, seq big-endian at seq_byte. Lengths above
+ * 65535 (payloads beyond a jumbo frame) are clamped to 65535. */
+typedef struct mixer_wire {
+    uint8_t eth_dst[6];
+    uint8_t eth_src[6];   /* zero: the NIC fills it on transmit (src/config.h:11) */
+    uint32_t ip_src;      /* host byte order */
+    uint32_t ip_dst;
+    uint16_t udp_port;
+    uint16_t reserved;
+} mixer_wire;
+void mixer_default_wire(mixer_wire* w); /* a0:88:c2:0d:5e:28, 10.0.0.1 -> 10.0.0.2, port 4096 */
+int mixer_build_header(const mixer_array_cfg* cfg, const mixer_wire* w, uint32_t seq, uint8_t* hdr);
+/* Header template used by mixer_synth_packets from now on (process-wide). */
+int mixer_synth_set_wire(const mixer_wire* w);
 
 #ifdef __cplusplus
 }

@@ -71,7 +71,7 @@ def test_synth_noise_free_matches_k_sky_restatement():
     for i in range(128):
         seq = 64 * 5 + i
         assert O.read_seq(np.ascontiguousarray(pk[i])) == seq
-        assert (pk[i, :48] == 0).all() and (pk[i, 52:64] == 0).all()
+        assert (pk[i, 42:48] == 0).all() and (pk[i, 52:64] == 0).all()  # reserved bytes
         exp = O.sky_payload(seq % 64, 8, 16, 32, 0.3, -0.45)
         assert np.array_equal(pk[i, 64:], exp), i
 
@@ -88,3 +88,38 @@ def test_synth_noise_is_seeded_and_gaussian():
     # noise is keyed by absolute (element, channel, sample): a later start reproduces the same packets
     e = M.synth_packets(c, sky, seed=7, seq0=32, npkts=224)
     assert np.array_equal(e, a[32:])
+
+
+def test_wire_header_is_the_transmitters():
+    """eth/ip/udp fields of src/tx_host.cu:69-79 with the src/config.h:53-58 defaults; the IPv4
+    checksum is pinned by its defining property (one's-complement sum of the header = 0xFFFF)."""
+    c = M.default_cfg()
+    h = M.build_header(c, 0x01020304)
+    assert bytes(h[0:6]) == bytes([0xa0, 0x88, 0xc2, 0x0d, 0x5e, 0x28]) and not h[6:12].any()
+    assert bytes(h[12:14]) == b"\x08\x00"
+    ip = h[14:34]
+    assert ip[0] == 0x45 and ip[8] == 64 and ip[9] == 17
+    assert int.from_bytes(bytes(ip[2:4]), "big") == 8256 - 14
+    assert bytes(ip[12:16]) == bytes([10, 0, 0, 1]) and bytes(ip[16:20]) == bytes([10, 0, 0, 2])
+    s = sum(int.from_bytes(bytes(ip[i:i + 2]), "big") for i in range(0, 20, 2))
+    while s >> 16:
+        s = (s & 0xFFFF) + (s >> 16)
+    assert s == 0xFFFF
+    udp = h[34:42]
+    assert int.from_bytes(bytes(udp[0:2]), "big") == 4096 == int.from_bytes(bytes(udp[2:4]), "big")
+    assert int.from_bytes(bytes(udp[4:6]), "big") == 8256 - 34 and not udp[6:8].any()
+    assert bytes(h[48:52]) == bytes([1, 2, 3, 4])
+    assert not h[42:48].any() and not h[52:64].any()
+    # a planted mistake in the checksum is caught by the same property
+    bad = ip.copy()
+    bad[10] ^= 1
+    s = sum(int.from_bytes(bytes(bad[i:i + 2]), "big") for i in range(0, 20, 2))
+    while s >> 16:
+        s = (s & 0xFFFF) + (s >> 16)
+    assert s != 0xFFFF
+    # synthetic packets carry that header with their own seq
+    sky = M.default_sky(sigma=0.0)
+    pk = M.synth_packets(c, sky, seed=1, seq0=513, npkts=3)
+    for k in range(3):
+        exp = M.build_header(c, 513 + k)
+        assert np.array_equal(pk[k, :64], exp)
