@@ -65,6 +65,35 @@ def test_reference_peek32_reads_corr32_layout():
     assert len([ln for ln in out.splitlines() if len(ln) == 2 * NG]) == NG  # the 32-row ASCII map
 
 
+def test_peek_viewer_reads_corr32_and_respects_writer_flag():
+    """The package viewer parses the same region: auto channel pick (brightest off-centre pixel,
+    src/peek32.cpp:30-39), the same ASCII map as the reference viewer where that is built, and
+    it never returns a frame while the writer flag (bit 63 of write_seq) is set."""
+    from mixer_amd import peek
+    img, vmax, vmin = G["img"], G["vmax"], G["vmin"]
+    region = corr32_region(img, vmax, vmin, write_seq=7)
+    ws, im, mx, mn = peek.read_frame(region.tobytes())
+    assert ws == 7 and np.array_equal(im, img.astype(np.float32)) and np.array_equal(mx, vmax.astype(np.float32))
+    yy, xx = np.mgrid[0:NG, 0:NG]
+    off = (xx - NG // 2) ** 2 + (yy - NG // 2) ** 2 > 4
+    brightest = [float(np.where(off, img[c], -np.inf).max()) for c in range(C)]
+    assert peek.pick_channel(im) == int(np.argmax(brightest))
+    text = peek.render(ws, im, mx, mn, ch=1)
+    assert len([ln for ln in text.splitlines() if len(ln) == 2 * NG]) == NG
+    if os.path.exists(PEEK) and not os.path.exists("/dev/shm/corr32"):
+        try:
+            region.tofile("/dev/shm/corr32")
+            ref = subprocess.run([PEEK, "1"], capture_output=True, text=True, timeout=30).stdout
+        finally:
+            os.unlink("/dev/shm/corr32")
+        rows = lambda t: [ln for ln in t.splitlines() if len(ln) == 2 * NG]
+        assert rows(ref) == rows(text)
+    busy = corr32_region(img, vmax, vmin, write_seq=(1 << 63) | 8)
+    with pytest.raises(TimeoutError):
+        peek.read_frame(busy.tobytes(), retries=3)
+    assert peek.read_frame(corr32_region(img, vmax, vmin, write_seq=0).tobytes()) is None
+
+
 @pytest.mark.gpu
 def test_hip_path_matches_golden():
     M.check(M.lib().mixer_set_device(0))
