@@ -428,9 +428,9 @@ def main():
     # HBM bytes per correlator launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this same command (scaled by a calibration copy); only valid for the launch shape profiled
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "xcorr_fold8_256el_128ch_8192s_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "xcorr_quad_256el_128ch_8192s_traffic.json")
     tun = M.get_tuning(cfg)
-    if args.config == "cfg2" and dom == "xcorr" and tun.staging == 1 and tun.prefetch == 4 and os.path.exists(tfile):
+    if args.config == "cfg2" and dom == "xcorr" and tun.staging == 1 and tun.prefetch == 3 and os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = int(json.load(fh)["traffic_bytes"])
     out = {
