@@ -1,15 +1,17 @@
-// corr_quad.h -- the QUAD correlator for 256-element arrays staged straight from the heaps.
+// corr_quad.h -- the QUAD correlator: arrays of NB % 4 == 0 element blocks (N = 128, 256, 384,
+// ..., 1024) staged straight from the heaps.
 //
 // Staging from the heaps, a one-channel workgroup fetches 32-byte runs: every 128-byte line of
 // a heap plane holds 4 channels, so each of its reads uses a quarter of a line and the gather is
 // bound by the number of L2 requests. Here a workgroup owns 4 adjacent channels (one line per
-// element and plane) and a quarter of the 8-block triangle, so each staged request is a whole
-// line and the register budget (5 tiles per wave) is unchanged:
-//   item 0 / 1  triangle of blocks 0-3 / 4-7 (10 tiles per channel): wave (ch, s) runs the fold
-//               pair (s, 3 - s) of channel ch (5 tiles)
-//   item 2 / 3  rectangle rows 4-5 / 6-7 x columns 0-3 (8 tiles per channel): wave (ch, s) runs
-//               row s of channel ch against the 4 columns (4 tiles)
-// Per channel the four items cover the 36 tiles of the triangle exactly once.
+// element and plane) and one item of the triangle, so each staged request is a whole line and the
+// register budget (5 tiles per wave) is unchanged. Items (a per-size list, see quad_items_for):
+//   (d0, -1)  4-block diagonal triangle at blocks d0..d0+3 (10 tiles per channel): wave (ch, s)
+//             runs the fold pair (s, 3 - s) of channel ch (5 tiles)
+//   (r0, c0)  rectangle rows r0, r0+1 x columns c0..c0+3 below the diagonal (8 tiles per
+//             channel): wave (ch, s) runs row s of channel ch against the 4 columns (4 tiles)
+// Per channel the items cover the lower triangle exactly once (N = 256: triangles 0-3, 4-7 and
+// rectangles rows 4-5, 6-7 x columns 0-3).
 #pragma once
 #include "corr_impl.h"
 
@@ -188,19 +190,20 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
     }
 }
 
-// grid: 4 items x ceil(C / 4) quads x n_slots (XCD-mapped: consecutive quads of a slot on one
-// XCD, the 4 items of a quad adjacent so they share its lines in L2), 512 threads.
+// grid: n_items x ceil(C / 4) quads x n_slots (XCD-mapped: the items of a quad adjacent and
+// consecutive quads of a slot on one XCD, so they share the quad's lines in L2), 512 threads.
 template <int V>
 __global__ __launch_bounds__(512)
-void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap, int C,
-                  int n_tb, int n_slots, int accumulate) {
-    constexpr int NT = 36;
+void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap,
+                  const int2* __restrict__ items, int n_items, int nb, int C, int n_tb,
+                  int n_slots, int accumulate) {
+    const int NT = nb * (nb + 1) / 2;
     __shared__ __attribute__((aligned(16))) uint8_t lds0[QuadLds<192>::BYTES];
     __shared__ __attribute__((aligned(16))) uint8_t lds1[QuadLds<192>::BYTES];
-    const int nq = (C + 3) / 4, total = 4 * nq * n_slots;
+    const int nq = (C + 3) / 4, total = n_items * nq * n_slots;
     const int logical = xcd_logical(blockIdx.x, total);
     if (logical >= total) return;  // uniform over the workgroup
-    const int item = logical & 3, q = (logical >> 2) % nq, slot = (logical >> 2) / nq;
+    const int item = logical % n_items, q = (logical / n_items) % nq, slot = logical / n_items / nq;
     const int kb0 = (int)((long)n_tb * slot / n_slots);
     const int kb1 = (int)((long)n_tb * (slot + 1) / n_slots);
     if (kb1 <= kb0) return;
@@ -212,8 +215,9 @@ void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
     const int n_kb = kb1 - kb0;
     const bool chk = wrap != nullptr;
     unsigned saw = 0;
-    if (item < 2) {
-        const int d0 = 4 * item;
+    const int2 it = items[item];
+    if (it.y < 0) {
+        const int d0 = it.x;
         auto gblk = [=](int b) { return d0 + b; };
         if (s == 0)
             quad_wave<V, 4, 3, 0, 4, 0, 0, 1>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
@@ -222,8 +226,8 @@ void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
             quad_wave<V, 4, 2, 0, 3, 1, 0, 2>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
                                               accumulate, chk, saw);
     } else {
-        const int r0 = item == 2 ? 4 : 6;  // LDS slots 0, 1: rows r0, r0 + 1; slots 2..5: columns 0..3
-        auto gblk = [=](int b) { return b < 2 ? r0 + b : b - 2; };
+        const int r0 = it.x, c0 = it.y;  // LDS slots 0, 1: rows r0, r0 + 1; 2..5: columns c0..c0+3
+        auto gblk = [=](int b) { return b < 2 ? r0 + b : c0 + b - 2; };
         if (s == 0)
             quad_wave<V, 6, 0, 2, 6, -1, 0, 0>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
                                                accumulate, chk, saw);
@@ -237,9 +241,10 @@ void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
 template <int V>
 inline hipError_t launch_quad_t(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
                                 uint32_t* wrap, int C, int n_tb, int accumulate) {
-    const int wgs = 4 * ((C + 3) / 4) * p.n_slots;
+    const int wgs = p.n_quad * ((C + 3) / 4) * p.n_slots;
     hipLaunchKernelGGL((k_xcorr_quad<V>), dim3((wgs + 7) / 8 * 8), dim3(512), 0, s, src, acc, wrap,
-                       C, n_tb, p.n_slots, accumulate);
+                       (const int2*)p.d_quad, p.n_quad, p.n_elem / 32, C, n_tb, p.n_slots,
+                       accumulate);
     return hipGetLastError();
 }
 

@@ -12,6 +12,9 @@ namespace mixer {
 
 int hip_status(hipError_t e);
 int plan_slots(const mixer_array_cfg& c);
+// the correlator layout contexts use for this geometry and tuning (XcorrPlan::Kind)
+int plan_kind(const mixer_array_cfg& c);
+std::vector<std::pair<int, int>> quad_items_for(int nb);
 // contexts correlate straight from the heaps (no corner-turn pass) under the tuning in effect
 bool plan_fused(const mixer_array_cfg& c);
 std::vector<std::pair<int, int>> sblocks_for(int nb);

@@ -6,9 +6,10 @@
 namespace mixer {
 
 // How the correlator covers the lower triangle of one channel.
-//   QUAD     256-element arrays staged straight from the heaps (prefetch 1 or 3): one
-//            workgroup per (4 adjacent channels, quarter of the triangle) so that every staged
-//            load is a whole 128-byte line of a heap plane.
+//   QUAD     arrays of a multiple of 4 blocks (N = 128, 256, 384, ..., 1024) staged straight
+//            from the heaps (prefetch 1 or 3): one workgroup per (4 adjacent channels, item of
+//            the triangle: a 4-block diagonal triangle or a 2x4-tile rectangle) so that every
+//            staged load is a whole 128-byte line of a heap plane.
 //   FOLD     arrays of <= 8 blocks of 32 elements: one workgroup per (channel, slot), one wave
 //            per folded row pair (r, nb-1-r); balanced (nb+1 tiles per wave).
 //   BLOCKED  larger arrays: one workgroup per 4x8-tile super-block, one wave per 2x2 tile
@@ -24,6 +25,8 @@ struct XcorrPlan {
     int fused;            // contexts: 1 = correlate straight from the heaps, 0 = via tiles
     int n_tasks;          // BLOCKED only
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
+    int n_quad;           // QUAD items: int2 (d0, -1) triangle or (r0, c0) rectangle
+    const void* d_quad;
     int n_segs;           // integrate work items
     const void* d_segs;   // int4 (d, j0, j1, 0): tiles (J + d, J), j0 <= J < j1
 };

@@ -50,7 +50,10 @@ int plan_slots(const mixer_array_cfg& c) {
     const mixer_tuning t = tuning_for(c.n_elem);
     if (t.slots > 0) return t.slots < n_tb ? t.slots : n_tb;
     long units;  // workgroups per slot
-    if (nb <= 8) {
+    const int kind = plan_kind(c);
+    if (kind == XcorrPlan::QUAD) {
+        units = (long)quad_items_for(nb).size() * ((c.n_chan + 3) / 4);
+    } else if (nb <= 8) {
         units = c.n_chan;
         // small folds run several workgroups per CU
         if (nb <= 5) units = (units + 3) / 4;
@@ -67,6 +70,30 @@ int plan_slots(const mixer_array_cfg& c) {
 bool plan_fused(const mixer_array_cfg& c) {
     const mixer_tuning t = tuning_for(c.n_elem);
     return t.staging != 2;
+}
+
+int plan_kind(const mixer_array_cfg& c) {
+    const int nb = c.n_elem / 32;
+    const mixer_tuning t = tuning_for(c.n_elem);
+    const int pf = t.prefetch > 0 ? t.prefetch : 3;
+    // straight from the heaps with register staging, arrays of 4 or 8 blocks use 4-channel QUAD
+    // items (256 x 128 x 8192: 0.216 vs 0.25 ms for the per-channel fold). Larger arrays stay
+    // on the per-channel BLOCKED layout: their QUAD rectangles move twice the bytes through L2
+    // per tile and measured slower (1024 x 256 x 4096: 3.30 vs 2.80 ms). The direct-to-LDS
+    // variants keep the per-channel layouts.
+    if (plan_fused(c) && (nb == 4 || nb == 8) && (pf == 1 || pf == 3)) return XcorrPlan::QUAD;
+    return nb <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
+}
+
+// QUAD items (nb % 4 == 0): the 4-block diagonal triangles (d0 = 4k, -1) and below them the
+// rectangles (r0, c0): rows r0, r0 + 1 (r0 = 4k, 4k + 2) x columns c0..c0+3 (c0 = 4j, j < k).
+std::vector<std::pair<int, int>> quad_items_for(int nb) {
+    std::vector<std::pair<int, int>> v;
+    for (int k = 0; k < nb / 4; ++k) v.push_back({4 * k, -1});
+    for (int k = 1; k < nb / 4; ++k)
+        for (int r0 = 4 * k; r0 < 4 * k + 4; r0 += 2)
+            for (int j = 0; j < k; ++j) v.push_back({r0, 4 * j});
+    return v;
 }
 
 // Integrate work items: each tile diagonal d (tiles (J + d, J)) in segments of at most 8 tiles.
@@ -96,6 +123,7 @@ namespace {
 std::mutex g_cache_mtx;
 std::map<std::pair<int, int>, void*> g_sblock_cache;  // (device, nb) -> int2 list
 std::map<std::pair<int, int>, void*> g_seg_cache;     // (device, nb) -> int4 list
+std::map<std::pair<int, int>, void*> g_quad_cache;    // (device, nb) -> int2 list
 std::map<std::pair<int, int>, void*> g_tw_cache;      // (device, ng) -> float2 twiddles
 std::map<std::pair<int, int>, bool> g_lds_set;        // (device, ng) -> attribute set
 }  // namespace
@@ -135,14 +163,28 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
         }
         p->d_segs = it->second;
     }
-    if (nb <= 8) {
-        // 256-element arrays from the heaps: 4-channel QUAD items (whole-line loads); the
-        // direct-to-LDS variants keep the per-channel FOLD layout
-        const bool quad = nb == 8 && p->fused && (p->prefetch == 1 || p->prefetch == 3);
-        p->kind = quad ? XcorrPlan::QUAD : XcorrPlan::FOLD;
-        return MIXER_OK;
+    p->kind = (XcorrPlan::Kind)plan_kind(c);
+    p->n_quad = 0;
+    p->d_quad = nullptr;
+    if (nb % 4 == 0) {
+        const auto qi = quad_items_for(nb);
+        p->n_quad = (int)qi.size();
+        std::lock_guard<std::mutex> lk(g_cache_mtx);
+        auto key = std::make_pair(current_device(), nb);
+        auto it = g_quad_cache.find(key);
+        if (it == g_quad_cache.end()) {
+            std::vector<int> h;
+            for (auto& q : qi) { h.push_back(q.first); h.push_back(q.second); }
+            void* d = nullptr;
+            int st = hip_status(hipMalloc(&d, h.size() * sizeof(int)));
+            if (st) return st;
+            st = hip_status(hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (st) return st;
+            it = g_quad_cache.emplace(key, d).first;
+        }
+        p->d_quad = it->second;
     }
-    p->kind = XcorrPlan::BLOCKED;
+    if (nb <= 8) return MIXER_OK;
     auto sb = sblocks_for(nb);
     p->n_tasks = (int)sb.size();
     std::lock_guard<std::mutex> lk(g_cache_mtx);
@@ -297,7 +339,8 @@ int mixer_xcorr(void* stream, const mixer_array_cfg* c, const int8_t* d_tiles, i
     XcorrPlan p;
     if ((st = get_plan(*c, &p))) return st;
     if (n_tb < p.n_slots) return MIXER_ERR_ARG;
-    if (p.kind == XcorrPlan::QUAD) p.kind = XcorrPlan::FOLD;  // tiles: the per-channel layout
+    if (p.kind == XcorrPlan::QUAD)  // tiles: the per-channel layouts
+        p.kind = c->n_elem / 32 <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
     return hip_status(launch_xcorr((hipStream_t)stream, p, xsrc_tiles(d_tiles, c->n_elem, n_tb),
                                    d_acc, nullptr, c->n_chan, n_tb, accumulate ? 1 : 0));
 }

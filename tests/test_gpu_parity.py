@@ -351,7 +351,8 @@ def test_autotune_sets_a_valid_plan():
 
 # ---------------------------------------------------------------- fused corner-turn + correlate
 @pytest.mark.parametrize("N,C,n_tb,parts", [
-    (256, 5, 8, 1), (256, 3, 12, 3), (32, 3, 3, 1), (64, 9, 4, 1), (96, 2, 2, 1), (160, 3, 5, 1),
+    (256, 5, 8, 1), (256, 3, 12, 3), (128, 6, 4, 1), (128, 9, 6, 2), (32, 3, 3, 1), (64, 9, 4, 1),
+    (96, 2, 2, 1), (160, 3, 5, 1),
     (224, 2, 3, 1), (384, 2, 2, 1), (512, 3, 3, 1), (1024, 1, 2, 1), (640, 1, 4, 2),
 ])
 def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
