@@ -180,7 +180,7 @@ def serve_arrays(rate_gbps, seed=0x4D495835, n=8, dump_hz=20.0, chunks_per_integ
     return cfgs, fs
 
 
-def bench_serve(rates=(50, 100, 250, 500), duration_s=6.5, budget=200 << 30):
+def bench_serve(rates=(50, 100, 250, 500, 650), duration_s=6.5, budget=200 << 30):
     out = []
     for r in rates:
         cfgs, fs = serve_arrays(r)
@@ -194,7 +194,8 @@ def bench_serve(rates=(50, 100, 250, 500), duration_s=6.5, budget=200 << 30):
              "late_submits": int(rep.late_submits), "p50_ms": round(rep.lat_p50_ms, 3),
              "p99_ms": round(rep.lat_p99_ms, 3), "max_ms": round(rep.lat_max_ms, 3),
              "samples_per_integration": cfgs[0].k_integ, "dump_hz": round(fs / cfgs[0].k_integ, 2),
-             "hbm_used_gib": round(st.hbm_used / 2**30, 1)}
+             "hbm_used_gib": round(st.hbm_used / 2**30, 1),
+             "correlator_launches": int(st.launches)}
         log(f"[bench] serve {r} GB/s: achieved {d['achieved_gbps']} GB/s, {d['integrations']} dumps, "
             f"p50 {d['p50_ms']} ms p99 {d['p99_ms']} ms (max {d['max_ms']}), late {d['late_submits']}")
         out.append(d)

@@ -74,7 +74,8 @@ class BatchInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("hbm_budget", "hbm_used", "admitted", "rejected", "chunks",
                                           "bytes_in", "integrations", "lat_n")] + \
-               [(n, C.c_double) for n in ("lat_p50_ms", "lat_p99_ms", "lat_max_ms", "lat_mean_ms")]
+               [(n, C.c_double) for n in ("lat_p50_ms", "lat_p99_ms", "lat_max_ms", "lat_mean_ms")] + \
+               [("launches", C.c_uint64)]
 
 
 class ServeSpec(C.Structure):
@@ -164,6 +165,7 @@ SIGNATURES = [
     ("mixer_server_submit", I32, [P, I32, P, I32, U64, P]),
     ("mixer_server_token_wait", I32, [P, I32, U64]),
     ("mixer_server_drain", I32, [P]),
+    ("mixer_server_set_coalescing", I32, [P, I32]),
     ("mixer_server_get_stats", I32, [P, P]),
     ("mixer_server_reset_latency", None, [P]),
     ("mixer_server_read_image", I32, [P, I32, P, P, P, P]),
@@ -609,6 +611,11 @@ class Server:
 
     def drain(self):
         check(lib().mixer_server_drain(self.h), "mixer_server_drain")
+
+    def set_coalescing(self, on: bool):
+        """Hold chunks that arrive while their array's previous launch runs and correlate the
+        contiguous ones in one launch (default on)."""
+        check(lib().mixer_server_set_coalescing(self.h, 1 if on else 0), "mixer_server_set_coalescing")
 
     def stats(self) -> ServerStats:
         s = ServerStats()

@@ -303,7 +303,14 @@ int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed);
  * when the chunk completes an integration, integrates + images it and copies the image cube to
  * pinned host memory. A completion thread timestamps each such dump: latency = dump complete on
  * the host - arrival of the integration's last chunk (steady clock).
- * Arrays are spread over n_streams compute streams (0 = 4). */
+ * Arrays are spread over n_streams compute streams (0 = 1: correlator kernels of different
+ * arrays running side by side spread each other's workgroups over the XCDs and lose the
+ * per-channel L2 reuse; 8 arrays of 256..1024 elements at 500 GB/s offered are served at
+ * 506 GB/s with one stream, 443 GB/s with four). A chunk that arrives while its
+ * array's previous correlator launch is still running is held, and the chunks contiguous with
+ * it join it in one longer launch (never across an integration); held chunks are launched when
+ * a later chunk finds the array idle, when its integration completes, or by drain/token_wait.
+ * Submission is single-threaded per array. */
 typedef struct mixer_server mixer_server;
 typedef struct mixer_server_stats {
     uint64_t hbm_budget;    /* bytes */
@@ -315,6 +322,7 @@ typedef struct mixer_server_stats {
     uint64_t integrations;  /* dumps completed on the host */
     uint64_t lat_n;         /* latency samples */
     double lat_p50_ms, lat_p99_ms, lat_max_ms, lat_mean_ms;
+    uint64_t launches;      /* correlator launches (fewer than updates when chunks coalesce) */
 } mixer_server_stats;
 
 int mixer_server_create(int device, uint64_t hbm_budget, int n_streams, mixer_server** out);
@@ -335,6 +343,8 @@ int mixer_server_input(mixer_server* srv, int array_id, uint8_t** d_ring, int* n
 int mixer_server_submit(mixer_server* srv, int array_id, const uint8_t* d_heaps, int n_tb,
                         uint64_t arrival_ns, uint64_t* token);
 int mixer_server_token_wait(mixer_server* srv, int array_id, uint64_t token);
+/* Chunk coalescing on (1, the default) or off (0: every chunk is launched when submitted). */
+int mixer_server_set_coalescing(mixer_server* srv, int enable);
 /* Wait until everything submitted is dumped. */
 int mixer_server_drain(mixer_server* srv);
 int mixer_server_get_stats(mixer_server* srv, mixer_server_stats* st);

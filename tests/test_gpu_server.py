@@ -119,3 +119,43 @@ def test_serve_run_poisson_arrivals():
         assert np.abs(slot.view(np.int8).astype(np.int16)).max() <= 127
         r, i = O.v_full_from_heaps(slot, n_tb, c.n_elem, c.n_chan, 32)
         assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
+
+
+@pytest.mark.parametrize("coalesce,staging", [(True, 0), (False, 0), (True, 2)])
+def test_server_coalesced_chunks_exact(coalesce, staging):
+    """Chunks submitted while their array's previous launch still runs (here: queued behind a
+    long launch of a large array on the same stream) are held and correlated in fewer, longer
+    launches; the dumps stay bit-identical to the oracle. With coalescing off every update is
+    its own launch. staging 2 (separate corner-turn) launches a held run one update at a time
+    through the tile buffer."""
+    big = cfg_for(1024, 32, 8192, 8192)
+    small = cfg_for(512, 8, 128, 1024)  # 4 time blocks per chunk, 8 chunks per integration
+    M.set_tuning(512, staging=staging)
+    try:
+        with M.Server(64 << 30, n_streams=1) as S:
+            S.set_coalescing(coalesce)
+            hb = S.admit(big)
+            s = S.admit(small)
+            p, _, _ = S.input(hb)
+            n_upd = 3 * 8
+            hs = heaps(512, 8, 4 * n_upd)
+            ds = M.DeviceBuffer.from_array(hs)
+            tb = 512 * 64 * 8
+            for _ in range(4):
+                S.submit(hb, p, 256, arrival_ns=0)
+            toks = [S.submit(s, ds.ptr.value + k * 4 * tb, 4, arrival_ns=0) for k in range(n_upd)]
+            S.token_wait(s, toks[-1])
+            S.drain()
+            st = S.stats()
+            log(f"coalesce={coalesce}: {st.launches} launches for {4 + n_upd} updates")
+            assert st.integrations == 4 + 3
+            if coalesce and staging != 2:
+                assert st.launches < 4 + n_upd
+            else:
+                assert st.launches == 4 + n_upd
+            v, idx = S.dump(s)
+            assert idx == 3
+            r, i = O.v_full_from_heaps(np.ascontiguousarray(hs[64:]), 32, 512, 8, 32)
+            assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
+    finally:
+        M.set_tuning(512)
