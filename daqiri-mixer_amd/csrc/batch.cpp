@@ -22,7 +22,8 @@ struct mixer_batch {
     BatchJobDev* d_jobs = nullptr;
     int4* d_items = nullptr;
     int items_cap = 0, n_items = 0, n_wgs = 0;
-    int* d_counter = nullptr;
+    int* d_counter = nullptr;  // one per XCD queue
+    BatchQueues queues{};
     uint32_t* d_wrap = nullptr;
     int ct_total = 0, int_total = 0, img_total = 0, max_ng = 8;
     std::vector<void*> allocs;
@@ -73,7 +74,7 @@ int enqueue(mixer_batch* b) {
     MIXER_HIP(launch_batch_reset(b->s, b->d_counter));
     if (b->ct_total > 0)
         MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
-    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->n_items, b->d_counter, b->n_wgs));
+    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->queues, b->d_counter, b->n_wgs));
     MIXER_HIP(launch_batch_integrate(b->s, b->d_jobs, (int)b->jobs.size(), b->int_total, b->max_ng));
     MIXER_HIP(launch_batch_image(b->s, b->d_jobs, (int)b->jobs.size(), b->img_total, b->max_ng));
     return MIXER_OK;
@@ -84,8 +85,10 @@ int plan(mixer_batch* b) {
     MIXER_HIP(hipStreamSynchronize(b->s));
     destroy_graph(b);
     const int pack = b->tun.pack, tbt = b->tun.tb_target > 0 ? b->tun.tb_target : 128;
-    struct Item { int4 w; double cost; };
-    std::vector<Item> items;
+    // groups of items that read the same data: the super-blocks of one (job, channel, slot), or
+    // the fold items of 8 adjacent channels (two 128-byte heap lines)
+    struct Group { std::vector<int4> w; double cost = 0; };
+    std::vector<Group> groups;
     b->exec_ops = 0;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const mixer_array_cfg& c = b->cfg[j];
@@ -99,20 +102,40 @@ int plan(mixer_batch* b) {
             const double kb = (double)n_tb / J.n_slots;
             if (nb <= 8) {
                 const int g = pack == 1 ? 1 : batch_fold_groups(nb);
-                for (int c0 = 0; c0 < c.n_chan; c0 += g)
-                    items.push_back({make_int4((int)j, c0, 0, sl), kb * NT * std::min(g, c.n_chan - c0)});
+                for (int c0 = 0; c0 < c.n_chan; c0 += g) {
+                    if (c0 == 0 || c0 / 8 != (c0 - g) / 8) groups.emplace_back();
+                    groups.back().w.push_back(make_int4((int)j, c0, 0, sl));
+                    groups.back().cost += kb * NT * std::min(g, c.n_chan - c0);
+                }
             } else {
                 const auto sb = sblocks_for(nb);
-                for (int ch = 0; ch < c.n_chan; ++ch)
+                for (int ch = 0; ch < c.n_chan; ++ch) {
+                    groups.emplace_back();
                     for (size_t q = 0; q < sb.size(); ++q) {
                         const int tiles = sb[q].second < 0 ? 36 : sb[q].second + 8 <= sb[q].first ? 32 : 10;
-                        items.push_back({make_int4((int)j, ch, (int)q, sl), kb * tiles});
+                        groups.back().w.push_back(make_int4((int)j, ch, (int)q, sl));
+                        groups.back().cost += kb * tiles;
                     }
+                }
             }
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-    b->n_items = (int)items.size();
+    // largest groups first, each to the least loaded XCD queue
+    std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) { return x.cost > y.cost; });
+    std::vector<int4> queue[8];
+    double load[8] = {0};
+    for (const Group& g : groups) {
+        const int x = (int)(std::min_element(load, load + 8) - load);
+        load[x] += g.cost;
+        queue[x].insert(queue[x].end(), g.w.begin(), g.w.end());
+    }
+    std::vector<int4> h;
+    for (int x = 0; x < 8; ++x) {
+        b->queues.start[x] = (int)h.size();
+        b->queues.size[x] = (int)queue[x].size();
+        h.insert(h.end(), queue[x].begin(), queue[x].end());
+    }
+    b->n_items = (int)h.size();
     if (b->n_items > b->items_cap) {
         if (b->d_items) {
             (void)hipFree(b->d_items);
@@ -121,8 +144,6 @@ int plan(mixer_batch* b) {
         b->items_cap = b->n_items;
         MIXER_TRY(balloc(b, (void**)&b->d_items, sizeof(int4) * b->items_cap));
     }
-    std::vector<int4> h(b->n_items);
-    for (int i = 0; i < b->n_items; ++i) h[i] = items[i].w;
     MIXER_HIP(hipMemcpy(b->d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
     MIXER_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJobDev) * b->jobs.size(),
                         hipMemcpyHostToDevice));
@@ -160,7 +181,7 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
     if (t) b->tun = *t;
     else mixer_batch_default_tuning(&b->tun);
     MIXER_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
-    MIXER_TRY(balloc(b, (void**)&b->d_counter, sizeof(int)));
+    MIXER_TRY(balloc(b, (void**)&b->d_counter, 8 * sizeof(int)));
     MIXER_TRY(balloc(b, (void**)&b->d_wrap, sizeof(uint32_t)));
     MIXER_HIP(hipMemset(b->d_wrap, 0, sizeof(uint32_t)));
     MIXER_TRY(balloc(b, (void**)&b->d_jobs, sizeof(BatchJobDev) * n_jobs));

@@ -91,11 +91,16 @@ struct BatchJobDev {
     int fused;                     // 1: correlate straight from the heaps, 0: from the tiles
 };
 
-// Correlator work item: (job, first channel, super-block index or 0, time slot)
-hipError_t launch_batch_reset(hipStream_t s, int* counter);
+// Correlator work item: (job, first channel, super-block index or 0, time slot). The items are
+// split into one queue per XCD: queue x is items [start[x], start[x] + size[x]).
+struct BatchQueues {
+    int start[8];
+    int size[8];
+};
+hipError_t launch_batch_reset(hipStream_t s, int* counters);
 hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs);
+                              const BatchQueues& q, int* counters, int n_wgs);
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
                                   int max_ng);
 hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,

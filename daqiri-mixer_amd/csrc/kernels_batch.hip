@@ -16,8 +16,8 @@ __device__ __forceinline__ int job_of(const BatchJobDev* jobs, int n_jobs, int b
     return j;
 }
 
-__global__ void k_batch_reset(int* counter) {
-    if (threadIdx.x == 0) *counter = 0;
+__global__ void k_batch_reset(int* counters) {
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
 constexpr int kBatchCCH = 8;  // corner-turn channels per workgroup in batched launches
@@ -179,20 +179,35 @@ __device__ __forceinline__ WaveTiles blk_tiles_rt(int I0, int J0, int w, bool* a
 constexpr int kBatchLds = 48 * 1024;
 
 // Persistent correlator: each workgroup takes the next work item (job, first channel,
-// super-block, time slot) until the list is exhausted. Items are ordered by decreasing cost,
-// so the longest start first and the tail is made of short items.
+// super-block, time slot) from the queue of its XCD (dispatch places workgroup b on XCD b % 8)
+// and, once that queue is empty, from the other XCDs' queues in turn, until all are empty. A
+// queue holds whole channels, largest first, so the super-blocks that share a channel's element
+// blocks run together on one XCD and share its L2; the queues end with the short items.
 __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restrict__ jobs,
-                                                     const int4* __restrict__ items, int n_items,
-                                                     int* __restrict__ counter) {
+                                                     const int4* __restrict__ items, BatchQueues q,
+                                                     int* __restrict__ counters) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[kBatchLds];
     __shared__ int item_sh;
     const int w8 = threadIdx.x >> 6;
+    const int home = blockIdx.x & 7;
+    int probe = 0;  // thread 0: queues found empty so far (an empty queue stays empty)
     for (;;) {
-        if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            int it = -1;
+            for (; probe < 8; ++probe) {
+                const int x = (home + probe) & 7;
+                const int k = atomicAdd(&counters[x], 1);
+                if (k < q.size[x]) {
+                    it = q.start[x] + k;
+                    break;
+                }
+            }
+            item_sh = it;
+        }
         __syncthreads();
         const int it = item_sh;
         __syncthreads();
-        if (it >= n_items) break;  // every wave reaches this with the same value
+        if (it < 0) break;  // every wave reaches this with the same value
         const int4 w = items[it];
         const BatchJobDev& J = jobs[w.x];
         const int nb = J.nb, slot = w.w;
@@ -275,8 +290,8 @@ __global__ __launch_bounds__(256) void k_batch_image(const BatchJobDev* __restri
     image_body(J.G, J.tw, J.img, J.img + nimg, J.img + nimg + J.n_chan, J.ng, b - J.img_wg0, smem);
 }
 
-hipError_t launch_batch_reset(hipStream_t s, int* counter) {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+hipError_t launch_batch_reset(hipStream_t s, int* counters) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
     return hipGetLastError();
 }
 
@@ -286,8 +301,8 @@ hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int 
 }
 
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              int n_items, int* counter, int n_wgs) {
-    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, n_items, counter);
+                              const BatchQueues& q, int* counters, int n_wgs) {
+    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
     return hipGetLastError();
 }
 

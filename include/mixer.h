@@ -242,8 +242,9 @@ void* mixer_compute_stream(mixer_ctx* ctx);
  * (no reference counterpart: the reference correlates one fixed array per process). A batch
  * holds n_jobs arrays of any supported geometry; each run correlates one integration
  * (cfg.k_integ samples) of every job with (optionally) one corner-turn launch, ONE persistent correlator
- * launch whose workgroups pull (job, channel group, super-block, time slot) work items from an
- * atomic counter (longest first), one integrate + gridding launch and one image launch. The
+ * launch whose workgroups pull (job, channel group, super-block, time slot) work items from
+ * per-XCD queues (whole channels per queue, largest first; an idle workgroup takes from the
+ * other queues), one integrate + gridding launch and one image launch. The
  * launch sequence is captured once as a hipGraph and replayed. Tuning:
  *   pack       small arrays (<= 256 elements): channels packed per correlator workgroup:
  *              0 = as many as fit (8 waves), 1 = one channel per workgroup
