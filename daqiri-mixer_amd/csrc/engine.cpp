@@ -110,7 +110,9 @@ int take_event(mixer_ctx* x, hipEvent_t* e) {
         x->ev_pool.pop_back();
         return MIXER_OK;
     }
-    return hip_status(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    // completion only (the host reads nothing through these): no system-scope release, whose
+    // L2 writeback idles the compute stream for ~20 us before the next integration
+    return hip_status(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
 }
 
 // Resolve completed timing pairs (all of them if wait).
@@ -238,7 +240,7 @@ int dispatch_ring(mixer_ctx* x) {
 int new_token(mixer_ctx* x, hipStream_t s, uint64_t* token) {
     const uint64_t t = x->tok_next++;
     hipEvent_t& e = x->tok_ev[t % mixer_ctx::NTOK];
-    if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
     MIXER_HIP(hipEventRecord(e, s));
     if (token) *token = t;
     return MIXER_OK;

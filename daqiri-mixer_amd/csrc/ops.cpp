@@ -96,11 +96,15 @@ std::vector<std::pair<int, int>> quad_items_for(int nb) {
     return v;
 }
 
-// Integrate work items: each tile diagonal d (tiles (J + d, J)) in segments of at most 8 tiles.
+// Integrate work units: runs of at most kSegTiles tiles (J + d, J) along tile diagonal d. A
+// workgroup loads its tiles one after another, so short runs keep the longest chain short
+// (256 x 128: 43 -> 35 us going from 8 to 2 tiles; 1024 x 256: 660 -> 632 us).
+constexpr int kSegTiles = 2;
 std::vector<int4> segs_for(int nb) {
     std::vector<int4> v;
     for (int d = 0; d < nb; ++d)
-        for (int j0 = 0; j0 < nb - d; j0 += 8) v.push_back(make_int4(d, j0, std::min(j0 + 8, nb - d), 0));
+        for (int j0 = 0; j0 < nb - d; j0 += kSegTiles)
+            v.push_back(make_int4(d, j0, std::min(j0 + kSegTiles, nb - d), 0));
     return v;
 }
 

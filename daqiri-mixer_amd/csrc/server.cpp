@@ -285,7 +285,7 @@ int launch_run(mixer_server* S, SrvArray* a) {
     }
     for (uint64_t t = a->tok_launched + 1; t < a->tok_next; ++t) {
         hipEvent_t& e = a->tok[t % SrvArray::NTOK];
-        if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
         MIXER_HIP(hipEventRecord(e, a->s));
     }
     a->tok_launched = a->tok_next - 1;
