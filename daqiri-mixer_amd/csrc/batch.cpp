@@ -64,10 +64,13 @@ void destroy_graph(mixer_batch* b) {
     b->graph = nullptr;
 }
 
-// staging 0 (automatic): arrays of <= 8 element blocks straight from the heaps, larger ones
-// (whose element blocks are re-read by many super-blocks) from corner-turned tiles
+// staging 0 (automatic): every array straight from the heaps. With the per-XCD queues the
+// super-blocks of a channel share their element blocks in one L2, and the 64-array batch runs
+// 108 ms against 121 ms with corner-turned tiles for the arrays above 256 elements (whose
+// corner-turn launch alone took 25 ms).
 bool job_fused(const mixer_batch* b, int nb) {
-    return b->tun.staging == 1 ? true : b->tun.staging == 2 ? false : nb <= 8;
+    (void)nb;
+    return b->tun.staging != 2;
 }
 
 int enqueue(mixer_batch* b) {
