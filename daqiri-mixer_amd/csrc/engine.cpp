@@ -7,7 +7,11 @@
 // integration), s_pub carries dumps and image read-outs. Events order them:
 //   ring_free[b]  s_comp finished the corner-turn reading ring buffer b -> s_copy may refill b
 //   copy_done     s_copy filled the current V-update -> s_comp may corner-turn it
-//   integ_ev[b]   V/image buffer b complete -> s_pub may read it
+//   integ_ev[b]   V/image buffer b complete -> s_pub may read it. Recorded by the reader at
+//                 the tail of s_comp (under the context mutex, so the latest integration is
+//                 already enqueued there), not after every integration: an event record's
+//                 system-scope release idles s_comp for ~13 us, and most integrations are
+//                 never read.
 //   pub_done[b]   s_pub finished reading buffer b -> s_comp may overwrite it
 // The int32 accumulator is written only by s_comp, so it has a single writer by construction
 // (the reference serialises two compute streams with vacc_ev, src/rx_host_corr.cu:150,166).
@@ -181,7 +185,6 @@ int finish_integration(mixer_ctx* x) {
                                img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng));
         MIXER_TRY(time_end(x, 3, t0));
     }
-    MIXER_HIP(hipEventRecord(x->integ_ev[b], x->s_comp));
     hipEvent_t e;
     MIXER_TRY(take_event(x, &e));
     MIXER_HIP(hipEventRecord(e, x->s_comp));
@@ -485,6 +488,7 @@ int mixer_dump(mixer_ctx* x, float* host, int cumulative, uint64_t* integ_index)
         if (integ_index) *integ_index = n;
         if (n == 0) return MIXER_OK;
         const int b = (int)((n - 1) % 2);
+        MIXER_HIP(hipEventRecord(x->integ_ev[b], x->s_comp));
         MIXER_HIP(hipStreamWaitEvent(x->s_pub, x->integ_ev[b], 0));
         MIXER_HIP(hipMemcpyAsync(host, cumulative ? x->d_vcum : x->d_v[b], x->v_bytes,
                                  hipMemcpyDeviceToHost, x->s_pub));
@@ -505,6 +509,7 @@ int mixer_read_image(mixer_ctx* x, float* img, float* vmax, float* vmin, uint64_
         if (integ_index) *integ_index = n;
         if (n == 0) return MIXER_OK;
         const int b = (int)((n - 1) % 2);
+        MIXER_HIP(hipEventRecord(x->integ_ev[b], x->s_comp));
         MIXER_HIP(hipStreamWaitEvent(x->s_pub, x->integ_ev[b], 0));
         const float* d = x->d_img[b];
         if (img) MIXER_HIP(hipMemcpyAsync(img, d, x->img_floats * 4, hipMemcpyDeviceToHost, x->s_pub));
