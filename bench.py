@@ -227,7 +227,19 @@ def cpu_baseline(cfg, heaps, n_tb, budget_s=12.0):
     what = f"{reps} x all {C} channels" if nch == C else f"{nch} of {C} channels"
     return {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{what} of one {n_tb * 32}-sample integration ({cfg.n_elem} elements), "
-                      f"full-square reference math, {dt:.1f} s"}
+                      f"full-square reference math, {dt:.1f} s on {threads} of {os.cpu_count()} "
+                      f"host threads ({cpu_model()})"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def main():

@@ -22,8 +22,9 @@ struct mixer_batch {
     BatchJobDev* d_jobs = nullptr;
     int4* d_items = nullptr;
     int items_cap = 0, n_items = 0, n_wgs = 0;
-    int* d_counter = nullptr;  // one per XCD queue
-    BatchQueues queues{};
+    int* d_counter = nullptr;  // one per XCD queue: [0, 8) fold items, [8, 16) BLOCKED items
+    BatchQueues queues{}, queues_blk{};
+    int n_fold_items = 0, n_blk_items = 0;
     uint32_t* d_wrap = nullptr;
     int ct_total = 0, int_total = 0, img_total = 0, max_ng = 8;
     std::vector<void*> allocs;
@@ -77,7 +78,11 @@ int enqueue(mixer_batch* b) {
     MIXER_HIP(launch_batch_reset(b->s, b->d_counter));
     if (b->ct_total > 0)
         MIXER_HIP(launch_batch_corner_turn(b->s, b->d_jobs, (int)b->jobs.size(), b->ct_total));
-    MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->queues, b->d_counter, b->n_wgs));
+    if (b->n_blk_items)
+        MIXER_HIP(launch_batch_xcorr_blk(b->s, b->d_jobs, b->d_items, b->queues_blk, b->d_counter + 8,
+                                         b->n_wgs));
+    if (b->n_fold_items)
+        MIXER_HIP(launch_batch_xcorr(b->s, b->d_jobs, b->d_items, b->queues, b->d_counter, b->n_wgs));
     MIXER_HIP(launch_batch_integrate(b->s, b->d_jobs, (int)b->jobs.size(), b->int_total, b->max_ng));
     MIXER_HIP(launch_batch_image(b->s, b->d_jobs, (int)b->jobs.size(), b->img_total, b->max_ng));
     return MIXER_OK;
@@ -91,7 +96,7 @@ int plan(mixer_batch* b) {
     // groups of items that read the same data: the super-blocks of one (job, channel, slot), or
     // the fold items of 8 adjacent channels (two 128-byte heap lines)
     struct Group { std::vector<int4> w; double cost = 0; };
-    std::vector<Group> groups;
+    std::vector<Group> groups, groups_blk;  // fold jobs / BLOCKED jobs (separate launches)
     b->exec_ops = 0;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const mixer_array_cfg& c = b->cfg[j];
@@ -113,31 +118,38 @@ int plan(mixer_batch* b) {
             } else {
                 const auto sb = sblocks_for(nb);
                 for (int ch = 0; ch < c.n_chan; ++ch) {
-                    groups.emplace_back();
+                    groups_blk.emplace_back();
                     for (size_t q = 0; q < sb.size(); ++q) {
                         const int tiles = sb[q].second < 0 ? 36 : sb[q].second + 8 <= sb[q].first ? 32 : 10;
-                        groups.back().w.push_back(make_int4((int)j, ch, (int)q, sl));
-                        groups.back().cost += kb * tiles;
+                        groups_blk.back().w.push_back(make_int4((int)j, ch, (int)q, sl));
+                        groups_blk.back().cost += kb * tiles;
                     }
                 }
             }
         }
     }
-    // largest groups first, each to the least loaded XCD queue
-    std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) { return x.cost > y.cost; });
-    std::vector<int4> queue[8];
-    double load[8] = {0};
-    for (const Group& g : groups) {
-        const int x = (int)(std::min_element(load, load + 8) - load);
-        load[x] += g.cost;
-        queue[x].insert(queue[x].end(), g.w.begin(), g.w.end());
-    }
+    // largest groups first, each to the least loaded XCD queue; the BLOCKED queues, then the
+    // fold queues, in one item list
     std::vector<int4> h;
-    for (int x = 0; x < 8; ++x) {
-        b->queues.start[x] = (int)h.size();
-        b->queues.size[x] = (int)queue[x].size();
-        h.insert(h.end(), queue[x].begin(), queue[x].end());
-    }
+    auto build = [&](std::vector<Group>& gs, BatchQueues& bq) {
+        std::stable_sort(gs.begin(), gs.end(), [](const Group& x, const Group& y) { return x.cost > y.cost; });
+        std::vector<int4> queue[8];
+        double load[8] = {0};
+        for (const Group& g : gs) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            load[x] += g.cost;
+            queue[x].insert(queue[x].end(), g.w.begin(), g.w.end());
+        }
+        const size_t h0 = h.size();
+        for (int x = 0; x < 8; ++x) {
+            bq.start[x] = (int)h.size();
+            bq.size[x] = (int)queue[x].size();
+            h.insert(h.end(), queue[x].begin(), queue[x].end());
+        }
+        return (int)(h.size() - h0);
+    };
+    b->n_blk_items = build(groups_blk, b->queues_blk);
+    b->n_fold_items = build(groups, b->queues);
     b->n_items = (int)h.size();
     if (b->n_items > b->items_cap) {
         if (b->d_items) {
@@ -184,7 +196,7 @@ int create_batch(const mixer_array_cfg* cfgs, int n_jobs, int device, const mixe
     if (t) b->tun = *t;
     else mixer_batch_default_tuning(&b->tun);
     MIXER_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
-    MIXER_TRY(balloc(b, (void**)&b->d_counter, 8 * sizeof(int)));
+    MIXER_TRY(balloc(b, (void**)&b->d_counter, 16 * sizeof(int)));
     MIXER_TRY(balloc(b, (void**)&b->d_wrap, sizeof(uint32_t)));
     MIXER_HIP(hipMemset(b->d_wrap, 0, sizeof(uint32_t)));
     MIXER_TRY(balloc(b, (void**)&b->d_jobs, sizeof(BatchJobDev) * n_jobs));

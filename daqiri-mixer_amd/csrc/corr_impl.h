@@ -388,10 +388,13 @@ struct BlkLds {
 // One work item (I0, J0) of the BLOCKED correlator on 512 threads with local ids ltid:
 // J0 < 0 is the diagonal triangle of element blocks I0..I0+7, run with the 8-wave FOLD layout
 // (36 tiles, no idle wave); J0 >= 0 is the 4x8-tile super-block at rows I0.., columns J0.. .
-template <int NB, int V>
-__device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32_t* a,
-                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                         int I0, int J0, int ltid, bool chk, unsigned& saw) {
+// nb (the array's element blocks) only clamps the staged column blocks, so the batched kernel
+// runs the same wave programs for every array size.
+template <int V>
+__device__ __forceinline__ void blk_item_n(const int8_t* src, const XSrc& m, int32_t* a,
+                                           uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                           int I0, int J0, int nb, int ltid, bool chk,
+                                           unsigned& saw) {
     static_assert(FoldLds<8, V>::BYTES <= BlkLds<V>::BYTES, "fold stage must fit the blocked LDS");
     if (J0 < 0) {
         fold_item<8, V>(src, m, a, lds0, lds1, n_kb, accumulate, ltid, chk, saw, I0);
@@ -399,7 +402,7 @@ __device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32
     }
     auto gblk = [=](int s) {
         const int g = s < 4 ? I0 + s : J0 + (s - 4);
-        return g < NB ? g : NB - 1;  // columns past the array only feed EMPTY waves
+        return g < nb ? g : nb - 1;  // columns past the array only feed EMPTY waves
     };
     const int w = ltid >> 6;
     static_for<0, 8>([&](auto Wc) {
@@ -415,6 +418,13 @@ __device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32
                 blk_prog<V, 2, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
         }
     });
+}
+
+template <int NB, int V>
+__device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32_t* a,
+                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
+                                         int I0, int J0, int ltid, bool chk, unsigned& saw) {
+    blk_item_n<V>(src, m, a, lds0, lds1, n_kb, accumulate, I0, J0, NB, ltid, chk, saw);
 }
 
 template <int NB, int V>

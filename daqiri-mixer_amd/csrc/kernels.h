@@ -99,8 +99,11 @@ struct BatchQueues {
 };
 hipError_t launch_batch_reset(hipStream_t s, int* counters);
 hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
+// fold items (arrays of <= 8 element blocks) / BLOCKED items (more); counters: 8 ints each
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
                               const BatchQueues& q, int* counters, int n_wgs);
+hipError_t launch_batch_xcorr_blk(hipStream_t s, const BatchJobDev* jobs, const int4* items,
+                                  const BatchQueues& q, int* counters, int n_wgs);
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,
                                   int max_ng);
 hipError_t launch_batch_image(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,

@@ -1,7 +1,8 @@
 // kernels_batch.hip -- batched launches over many independent arrays ("jobs") of different
-// sizes: one corner-turn launch, one persistent correlator launch pulling work items from an
-// atomic counter, one integrate (+ gridding) launch and one image launch. The batch engine
-// captures the sequence once as a hipGraph and replays it.
+// sizes: one corner-turn launch, two persistent correlator launches pulling work items from
+// per-XCD queues (arrays of more than 8 element blocks on the blocked wave programs, the others
+// on a runtime-generic fold program), one integrate (+ gridding) launch and one image launch.
+// The batch engine captures the sequence once as a hipGraph and replays it.
 #include "corr_impl.h"
 #include "kernels_body.h"
 
@@ -17,7 +18,7 @@ __device__ __forceinline__ int job_of(const BatchJobDev* jobs, int n_jobs, int b
 }
 
 __global__ void k_batch_reset(int* counters) {
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+    if (threadIdx.x < 16) counters[threadIdx.x] = 0;
 }
 
 constexpr int kBatchCCH = 8;  // corner-turn channels per workgroup in batched launches
@@ -178,6 +179,29 @@ __device__ __forceinline__ WaveTiles blk_tiles_rt(int I0, int J0, int w, bool* a
 // most 32 KiB (e.g. 4 groups x 2 x 4 KiB at 64 elements), BLOCKED 2 x 24 KiB.
 constexpr int kBatchLds = 48 * 1024;
 
+// Take the next item index of this workgroup's XCD queue, or of the others' once it is empty;
+// -1 when every queue is empty (the same value for the whole workgroup). probe (thread 0):
+// queues found empty so far; an empty queue stays empty.
+__device__ __forceinline__ int next_item(const BatchQueues& q, int* counters, int home, int& probe,
+                                         int& item_sh) {
+    if (threadIdx.x == 0) {
+        int it = -1;
+        for (; probe < 8; ++probe) {
+            const int x = (home + probe) & 7;
+            const int k = atomicAdd(&counters[x], 1);
+            if (k < q.size[x]) {
+                it = q.start[x] + k;
+                break;
+            }
+        }
+        item_sh = it;
+    }
+    __syncthreads();
+    const int it = item_sh;
+    __syncthreads();
+    return it;
+}
+
 // Persistent correlator: each workgroup takes the next work item (job, first channel,
 // super-block, time slot) from the queue of its XCD (dispatch places workgroup b on XCD b % 8)
 // and, once that queue is empty, from the other XCDs' queues in turn, until all are empty. A
@@ -192,21 +216,7 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
     const int home = blockIdx.x & 7;
     int probe = 0;  // thread 0: queues found empty so far (an empty queue stays empty)
     for (;;) {
-        if (threadIdx.x == 0) {
-            int it = -1;
-            for (; probe < 8; ++probe) {
-                const int x = (home + probe) & 7;
-                const int k = atomicAdd(&counters[x], 1);
-                if (k < q.size[x]) {
-                    it = q.start[x] + k;
-                    break;
-                }
-            }
-            item_sh = it;
-        }
-        __syncthreads();
-        const int it = item_sh;
-        __syncthreads();
+        const int it = next_item(q, counters, home, probe, item_sh);
         if (it < 0) break;  // every wave reaches this with the same value
         const int4 w = items[it];
         const BatchJobDev& J = jobs[w.x];
@@ -270,6 +280,41 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
     }
 }
 
+// Persistent correlator for the arrays of more than 8 element blocks: the items are BLOCKED
+// super-blocks (diagonal 8-block triangles, full 4 x 8 super-blocks, the tail triangle) run by
+// the same wave programs as k_xcorr_blk (two register sets, loads two blocks ahead).
+__global__ __launch_bounds__(512) void k_batch_xcorr_blk(const BatchJobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ items,
+                                                         BatchQueues q,
+                                                         int* __restrict__ counters) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds0[BlkLds<3>::BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds1[BlkLds<3>::BYTES];
+    __shared__ int item_sh;
+    const int home = blockIdx.x & 7;
+    int probe = 0;
+    for (;;) {
+        const int it = next_item(q, counters, home, probe, item_sh);
+        if (it < 0) break;
+        const int4 w = items[it];
+        const BatchJobDev& J = jobs[w.x];
+        const int nb = J.nb, slot = w.w, c = w.y;
+        const int kb0 = (int)((long)J.n_tb * slot / J.n_slots);
+        const int kb1 = (int)((long)J.n_tb * (slot + 1) / J.n_slots);
+        if (kb1 > kb0) {  // uniform
+            const XSrc m = J.fused ? xsrc_heaps(J.heaps, J.n_elem, J.n_chan)
+                                   : xsrc_tiles(J.tiles, J.n_elem, J.n_tb);
+            const int2 sb = J.sblocks[w.z];
+            const int NT = nb * (nb + 1) / 2;
+            const int8_t* src = m.base + (size_t)c * m.chan + (size_t)kb0 * m.tb;
+            int32_t* a = J.acc + ((size_t)slot * J.n_chan + c) * NT * 2048;
+            unsigned saw = 0;
+            blk_item_n<3>(src, m, a, lds0, lds1, kb1 - kb0, 0, sb.x, sb.y, nb, threadIdx.x, false,
+                          saw);
+        }
+        __syncthreads();  // LDS reuse by the next item
+    }
+}
+
 __global__ __launch_bounds__(256) void k_batch_integrate(const BatchJobDev* __restrict__ jobs,
                                                          int n_jobs) {
     const int b = blockIdx.x;
@@ -303,6 +348,12 @@ hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int 
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
                               const BatchQueues& q, int* counters, int n_wgs) {
     hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_xcorr_blk(hipStream_t s, const BatchJobDev* jobs, const int4* items,
+                                  const BatchQueues& q, int* counters, int n_wgs) {
+    hipLaunchKernelGGL(k_batch_xcorr_blk, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
     return hipGetLastError();
 }
 

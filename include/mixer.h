@@ -241,8 +241,9 @@ void* mixer_compute_stream(mixer_ctx* ctx);
 /* ---- batched correlation of many independent arrays --------------------------------------
  * (no reference counterpart: the reference correlates one fixed array per process). A batch
  * holds n_jobs arrays of any supported geometry; each run correlates one integration
- * (cfg.k_integ samples) of every job with (optionally) one corner-turn launch, ONE persistent correlator
- * launch whose workgroups pull (job, channel group, super-block, time slot) work items from
+ * (cfg.k_integ samples) of every job with (optionally) one corner-turn launch, two persistent
+ * correlator launches (arrays above / up to 256 elements) whose workgroups pull (job, channel
+ * group, super-block, time slot) work items from
  * per-XCD queues (whole channels per queue, largest first; an idle workgroup takes from the
  * other queues), one integrate + gridding launch and one image launch. The
  * launch sequence is captured once as a hipGraph and replayed. Tuning:
