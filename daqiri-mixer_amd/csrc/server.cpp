@@ -306,8 +306,11 @@ int submit(mixer_server* S, SrvArray* a, const uint8_t* d, int n_tb, uint64_t ar
         }
     }
     const uint64_t t = a->tok_next++;
-    // hold the run while the array's previous launch is still running
-    const bool busy = S->coalesce && a->run_tb > 0 && hipEventQuery(a->ev_launch) == hipErrorNotReady;
+    // hold the run while the array's previous launch is still running, and while its tokens
+    // fit the event ring (an older token counts as recycled, i.e. complete)
+    const bool busy = S->coalesce && a->run_tb > 0 &&
+                      a->tok_next - 1 - a->tok_launched < (uint64_t)SrvArray::NTOK / 2 &&
+                      hipEventQuery(a->ev_launch) == hipErrorNotReady;
     if (!busy) MIXER_TRY(launch_run(S, a));
     if (token) *token = t;
     std::lock_guard<std::mutex> lk(S->m);

@@ -159,3 +159,31 @@ def test_server_coalesced_chunks_exact(coalesce, staging):
             assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
     finally:
         M.set_tuning(512)
+
+
+def test_server_holds_at_most_half_the_token_ring():
+    """An integration of 128 one-block chunks queued behind a long launch: the held run is
+    launched every 32 tokens (a token older than the 64-event ring would count as complete), and
+    the dump is exact."""
+    big = cfg_for(1024, 32, 8192, 8192)
+    small = cfg_for(512, 4, 32, 4096)  # 1 time block per chunk, 128 chunks per integration
+    with M.Server(64 << 30, n_streams=1) as S:
+        hb = S.admit(big)
+        s = S.admit(small)
+        p, _, _ = S.input(hb)
+        hs = heaps(512, 4, 128)
+        ds = M.DeviceBuffer.from_array(hs)
+        tb = 512 * 64 * 4
+        for _ in range(4):
+            S.submit(hb, p, 256, arrival_ns=0)
+        toks = [S.submit(s, ds.ptr.value + k * tb, 1, arrival_ns=0) for k in range(128)]
+        S.token_wait(s, toks[0])
+        S.token_wait(s, toks[40])
+        S.drain()
+        st = S.stats()
+        log(f"128 chunks: {st.launches - 4} launches")
+        assert 128 // 32 <= st.launches - 4 <= 128
+        v, idx = S.dump(s)
+        assert idx == 1
+        r, i = O.v_full_from_heaps(hs, 128, 512, 4, 32)
+        assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
