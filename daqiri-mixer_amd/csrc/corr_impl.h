@@ -75,7 +75,7 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
                 const int u = ltid + i * NTH;
                 if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
                     const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                    st.r[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
+                    st.r[i] = gld((const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1)));
                 }
             }
         };
@@ -125,7 +125,7 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
                 const int u = ltid + i * NTH;
                 if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
                     const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                    st.r[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
+                    st.r[i] = gld((const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1)));
                 }
             }
         };

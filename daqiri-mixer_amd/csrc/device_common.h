@@ -66,6 +66,20 @@ __device__ __forceinline__ v4i frag(const uint8_t* lds, int p, int B, int lrow, 
     return *(const v4i*)(lds + (lds_unit<NE>(p, B * 32 + lrow, kg) << 4));
 }
 
+// Loads and stores of device (global) memory through a generic pointer. A pointer read from
+// device memory (a job table) is generic to the compiler, and a plain access through it becomes a
+// FLAT instruction, which counts on the LDS counter too: every wait for an LDS fragment read
+// would then also wait for the in-flight global prefetch. Accessing through a global-address-
+// space pointer gives global loads and stores (every pointer passed here is device memory).
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, const T& v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
 // Copies a list of element blocks of one slab global -> registers -> LDS. NBLK blocks, each
 // 2 planes x 32 elements x 32 samples = 128 chunks of 16 B; NT threads with local ids ltid.
 template <int NE, int NBLK, int NT>
@@ -81,7 +95,7 @@ struct Stager {
             const int u = ltid + i * NT;
             if (CHUNKS % NT == 0 || u < CHUNKS) {
                 const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                r[i] = *(const v4i*)(slab + (size_t)p * NE * 32 + blk(b) * 1024 + eg * 16);
+                r[i] = gld((const v4i*)(slab + (size_t)p * NE * 32 + blk(b) * 1024 + eg * 16));
             }
         }
     }
@@ -107,8 +121,8 @@ __device__ __forceinline__ void store_tile(const v16i& x, int32_t* __restrict__ 
         const int row = 8 * (r >> 2) + 4 * kg + (r & 3);
         int32_t* p = dst + row * 32 + lrow;
         int v = x[r];
-        if (accumulate) v += *p;
-        *p = v;
+        if (accumulate) v += gld(p);
+        gst(p, v);
     }
 }
 

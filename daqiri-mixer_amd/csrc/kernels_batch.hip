@@ -81,7 +81,7 @@ __device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, con
             const int u = ltid + i * nth;
             if (ltid >= 0 && u < chunks) {
                 const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
-                st[i] = *(const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1));
+                st[i] = gld((const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1)));
             }
         }
     };

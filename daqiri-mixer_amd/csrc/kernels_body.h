@@ -31,8 +31,8 @@ __device__ __forceinline__ void corner_turn_body(const uint8_t* __restrict__ hea
         const int e = q >> 1, p = q & 1, c = w >> 1, h = w & 1;
         v[i] = (v4i){0, 0, 0, 0};
         if (c0 + c < c_end)
-            v[i] = *(const v4i*)(heaps + ((size_t)tb * N + e0 + e) * payload + (size_t)p * C * 32 +
-                                 (size_t)(c0 + c) * 32 + h * 16);
+            v[i] = gld((const v4i*)(heaps + ((size_t)tb * N + e0 + e) * payload + (size_t)p * C * 32 +
+                                    (size_t)(c0 + c) * 32 + h * 16));
     }
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
@@ -51,7 +51,7 @@ __device__ __forceinline__ void corner_turn_body(const uint8_t* __restrict__ hea
         if (c0 + cc >= c_end) continue;
         const int s = lane ^ ((cc * 2) & 15);
         const v4i x = *(const v4i*)(lds + (((p * CCH + cc) * 64 + s) << 4));
-        *(v4i*)(tiles + ((((size_t)(c0 + cc) * n_tb + tb) * 2 + p) * N + e0) * 32 + lane * 16) = x;
+        gst((v4i*)(tiles + ((((size_t)(c0 + cc) * n_tb + tb) * 2 + p) * N + e0) * 32 + lane * 16), x);
     }
     if (wrap) {
         __shared__ unsigned any;
@@ -121,8 +121,8 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
         const int32_t* t = ac + ((size_t)I * (I + 1) / 2 + J) * 2048 + i * 32 + jq;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            x[p] = *(const v4i*)(t + p * 1024);
-            for (int sl = 1; sl < n_slots; ++sl) x[p] += *(const v4i*)(t + sl * slot_stride + p * 1024);
+            x[p] = gld((const v4i*)(t + p * 1024));
+            for (int sl = 1; sl < n_slots; ++sl) x[p] += gld((const v4i*)(t + sl * slot_stride + p * 1024));
         }
     };
     auto put = [&](int row, int col, int p, float4 f) {  // V element (row, col..col+3) of plane p
