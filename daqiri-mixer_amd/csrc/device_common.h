@@ -8,6 +8,7 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace mixer {
 
@@ -78,6 +79,10 @@ __device__ __forceinline__ T gld(const T* p) {
 template <typename T>
 __device__ __forceinline__ void gst(T* p, const T& v) {
     *(__attribute__((address_space(1))) T*)p = v;
+}
+__device__ __forceinline__ void gadd(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long*)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Copies a list of element blocks of one slab global -> registers -> LDS. NBLK blocks, each

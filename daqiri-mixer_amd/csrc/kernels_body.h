@@ -127,11 +127,10 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
     };
     auto put = [&](int row, int col, int p, float4 f) {  // V element (row, col..col+3) of plane p
         const size_t off = ((size_t)c * 2 + p) * N * N + (size_t)row * N + col;
-        if (v_store) *(float4*)(v_store + off) = f;
+        const v4f fv = {f.x, f.y, f.z, f.w};
+        if (v_store) gst((v4f*)(v_store + off), fv);
         if (v_add) {
-            float4 o = *(float4*)(v_add + off);
-            o.x += f.x; o.y += f.y; o.z += f.z; o.w += f.w;
-            *(float4*)(v_add + off) = o;
+            gst((v4f*)(v_add + off), gld((const v4f*)(v_add + off)) + fv);
         }
     };
     v4i nxt[2];
@@ -174,12 +173,12 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
             if (!re && !im) continue;
             const int dy = R * d + q / W - (R - 1), dx = q % W - (gs - 1);
             const int y = dy < 0 ? dy + ng : dy, x = dx < 0 ? dx + ng : dx;
-            atomicAdd(&Gc[2 * (y * ng + x)], re);
-            atomicAdd(&Gc[2 * (y * ng + x) + 1], im);
+            gadd(&Gc[2 * (y * ng + x)], re);
+            gadd(&Gc[2 * (y * ng + x) + 1], im);
             if (d > 0) {
                 const int y2 = y ? ng - y : 0, x2 = x ? ng - x : 0;
-                atomicAdd(&Gc[2 * (y2 * ng + x2)], re);
-                atomicAdd(&Gc[2 * (y2 * ng + x2) + 1], (unsigned long long)(-(long long)im));
+                gadd(&Gc[2 * (y2 * ng + x2)], re);
+                gadd(&Gc[2 * (y2 * ng + x2) + 1], (unsigned long long)(-(long long)im));
             }
         }
     }
@@ -207,9 +206,9 @@ __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
     const int tid = threadIdx.x;
     unsigned long long* Gc = G + (size_t)c * 2 * nn;
     for (int q = tid; q < nn; q += 256) {
-        F0[q] = make_float2((float)(long long)Gc[2 * q], (float)(long long)Gc[2 * q + 1]);
-        Gc[2 * q] = 0ull;
-        Gc[2 * q + 1] = 0ull;
+        F0[q] = make_float2((float)(long long)gld(&Gc[2 * q]), (float)(long long)gld(&Gc[2 * q + 1]));
+        gst(&Gc[2 * q], 0ull);
+        gst(&Gc[2 * q + 1], 0ull);
     }
     __syncthreads();
     float2* src = F0;
@@ -237,7 +236,7 @@ __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
     for (int q = tid; q < nn; q += 256) {
         const int y = q / ng, x = q % ng;
         const float v = src[((y + half) % ng) * ng + (x + half) % ng].x;
-        im[q] = v;
+        gst(&im[q], v);
         mx = fmaxf(mx, v);
         mn = fminf(mn, v);
     }
@@ -250,8 +249,8 @@ __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
     if (tid == 0) {
         float a = red_mx[0], b = red_mn[0];
         for (int k = 1; k < 4; ++k) { a = fmaxf(a, red_mx[k]); b = fminf(b, red_mn[k]); }
-        vmax[c] = a;
-        vmin[c] = b;
+        gst(&vmax[c], a);
+        gst(&vmin[c], b);
     }
 }
 
