@@ -44,6 +44,11 @@ struct XSrc {
     __device__ __forceinline__ size_t off(int p, int e, int h) const {
         return (size_t)p * plane + (size_t)e * elem + h * 16;
     }
+    // the same within one time block (< 2^32 bytes: plane <= 8 KiB, elem <= 16 KiB, e < 1024), so
+    // a load is a uniform 64-bit base plus one 32-bit register
+    __device__ __forceinline__ uint32_t off32(int p, int e, int h) const {
+        return (uint32_t)p * (uint32_t)plane + (uint32_t)e * (uint32_t)elem + (uint32_t)h * 16u;
+    }
 };
 __host__ __device__ inline XSrc xsrc_tiles(const int8_t* tiles, int N, int n_tb) {
     const size_t slab = (size_t)N * 64;
