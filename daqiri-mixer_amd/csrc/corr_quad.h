@@ -38,15 +38,14 @@ struct QuadStager {
     // would keep a 64-bit address per load.
     template <typename GB>
     __device__ __forceinline__ void load(const int8_t* sl, const XSrc& m, GB gblk, int q4, int C) {
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)sl, (short)0, 0x7fffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rs = block_rsrc(sl);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int u = threadIdx.x + i * 512;
             const int el = u >> 4, p = (u >> 3) & 1, ch = (u >> 1) & 3, h = u & 1;
             const int c = q4 + ch < C ? q4 + ch : C - 1;  // ragged last quad: re-read a valid channel
             const uint32_t o = m.off32(p, gblk(el >> 5) * 32 + (el & 31), h) + (uint32_t)c * (uint32_t)m.chan;
-            r[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)o, 0, 0));
+            r[i] = bload(rs, o);
         }
     }
     __device__ __forceinline__ void store(uint8_t* L) const {

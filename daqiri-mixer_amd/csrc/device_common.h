@@ -80,6 +80,18 @@ template <typename T>
 __device__ __forceinline__ void gst(T* p, const T& v) {
     *(__attribute__((address_space(1))) T*)p = v;
 }
+// Buffer loads of one staged block: the block's base in a scalar descriptor and one 32-bit offset
+// register per load (the same offsets for every block), where a global load keeps a 64-bit
+// address per load. Every offset is below 2^31 (one time block of one array). Used for the QUAD
+// staging of whole 128-byte lines (0.215 -> 0.205 ms at 256 x 128 x 8192); the gathers of
+// 32-byte runs (fold, blocked) measured slower with them (1024 x 256 x 4096: 2.96 vs 2.85 ms)
+// and stay on global loads.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ v4i bload(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0));
+}
 __device__ __forceinline__ void gadd(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long*)p, v, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
