@@ -46,7 +46,9 @@ __global__ __launch_bounds__(256) void k_batch_corner_turn(const BatchJobDev* __
 // per block, like corr_wave with V = 1).
 // ------------------------------------------------------------------------------------------
 constexpr int kMaxT = 5;
-constexpr int kMaxStage = 3;  // 16-byte chunks staged per thread per block (NBLK*128 <= 3*512)
+// 16-byte chunks staged per thread per block: a fold group of NB <= 8 blocks has NB * 128 chunks
+// over 64 * fold_waves_rt(NB) threads, at most 2 each
+constexpr int kMaxStage = 2;
 
 struct WaveTiles {
     int ra, ra_g, a0, a0_g, na;
@@ -161,23 +163,9 @@ __device__ __forceinline__ WaveTiles fold_tiles_rt(int nb, int w, bool* active) 
     return t;
 }
 
-// The 2x2 tile block of wave w in BLOCKED super-block (I0, J0): LDS slots 0..3 hold the rows
-// I0..I0+3, slots 4..11 the columns J0..J0+7.
-__device__ __forceinline__ WaveTiles blk_tiles_rt(int I0, int J0, int w, bool* active) {
-    WaveTiles t{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int RS = 2 * (w / 4), CS = 2 * (w % 4);
-    const int Ib = I0 + RS, Jb = J0 + CS;
-    *active = Jb <= Ib;
-    if (!*active) return t;
-    const bool full = Jb + 1 < Ib;
-    t.ra = RS; t.ra_g = Ib; t.a0 = 4 + CS; t.a0_g = Jb; t.na = full ? 2 : 1;
-    t.rb = RS + 1; t.rb_g = Ib + 1; t.b0 = 4 + CS; t.b0_g = Jb; t.nb = 2;
-    return t;
-}
-
-// LDS of the persistent correlator: two stage buffers per channel group; packed folds need at
-// most 32 KiB (e.g. 4 groups x 2 x 4 KiB at 64 elements), BLOCKED 2 x 24 KiB.
-constexpr int kBatchLds = 48 * 1024;
+// LDS of the persistent fold correlator: two stage buffers per channel group, at most 32 KiB
+// (8 blocks: 2 x 16 KiB; packed folds e.g. 4 groups x 2 x 4 KiB at 64 elements).
+constexpr int kBatchLds = 32 * 1024;
 
 // Take the next item index of this workgroup's XCD queue, or of the others' once it is empty;
 // -1 when every queue is empty (the same value for the whole workgroup). probe (thread 0):
@@ -225,48 +213,21 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
         const int kb1 = (int)((long)J.n_tb * (slot + 1) / J.n_slots);
         const int NT = nb * (nb + 1) / 2;
         if (kb1 > kb0) {  // uniform
-            // One call site for every array size (a second inlined copy makes the allocator
-            // spill): fold channels and BLOCKED super-blocks differ only in these parameters.
+            // Fold items only (arrays of at most 8 element blocks; the larger ones run on
+            // k_batch_xcorr_blk): one call site for every array size, a channel group per item.
             const XSrc m = J.fused ? xsrc_heaps(J.heaps, J.n_elem, J.n_chan)
                                  : xsrc_tiles(J.tiles, J.n_elem, J.n_tb);
-            int nblk, buf, ltid, nth, c, row0, col0;
-            bool active = false, have = true;
-            WaveTiles wt;
-            uint8_t* l0 = lds;
-            if (nb <= 8) {
-                const int wpc = fold_waves_rt(nb), G = min(8 / wpc, J.ch_per_item);
-                const int g = w8 / wpc;
-                nblk = nb;
-                buf = nb * 32 * 64;
-                nth = 64 * wpc;
-                ltid = g < G ? (int)(threadIdx.x % nth) : -1;
-                c = w.y + (g < G ? g : 0);
-                have = g < G && c < J.n_chan;
-                if (!have) c = w.y;  // idle groups stage a valid slab and store nothing
-                wt = fold_tiles_rt(nb, w8 % wpc, &active);
-                l0 = lds + (g < G ? g : 0) * 2 * buf;
-                row0 = 0;
-                col0 = 4;  // identity block map
-            } else {
-                const int2 sb = J.sblocks[w.z];
-                nth = 512;
-                ltid = threadIdx.x;
-                c = w.y;
-                if (sb.y < 0) {  // diagonal triangle of blocks sb.x..sb.x+7: the 8-wave fold layout
-                    nblk = 8;
-                    buf = 8 * 32 * 64;
-                    wt = fold_tiles_rt(8, w8, &active);
-                    wt.ra_g += sb.x; wt.a0_g += sb.x; wt.rb_g += sb.x; wt.b0_g += sb.x;
-                    row0 = sb.x;
-                    col0 = sb.x + 4;
-                } else {
-                    nblk = 12;
-                    buf = 12 * 32 * 64;
-                    wt = blk_tiles_rt(sb.x, sb.y, w8, &active);
-                    row0 = sb.x;
-                    col0 = sb.y;
-                }
-            }
+            bool active = false;
+            const int wpc = fold_waves_rt(nb), G = min(8 / wpc, J.ch_per_item);
+            const int g = w8 / wpc;
+            const int nblk = nb, buf = nb * 32 * 64, nth = 64 * wpc;
+            const int ltid = g < G ? (int)(threadIdx.x % nth) : -1;
+            int c = w.y + (g < G ? g : 0);
+            const bool have = g < G && c < J.n_chan;
+            if (!have) c = w.y;  // idle groups stage a valid slab and store nothing
+            const WaveTiles wt = fold_tiles_rt(nb, w8 % wpc, &active);
+            uint8_t* l0 = lds + (g < G ? g : 0) * 2 * buf;
+            const int row0 = 0, col0 = 4;  // identity block map
             auto gblk = [=](int b) {
                 const int g = b < 4 ? row0 + b : col0 + (b - 4);
                 return g < nb ? g : nb - 1;  // columns past the array only feed idle waves
