@@ -296,7 +296,8 @@ def main():
         w0 = time.perf_counter()
         M.check(L.mixer_event_record(ev0, stream))
         for i in range(args.steps):
-            eng.submit_device(rings[i % 2], n_tb)
+            # the two rings are never rewritten: no completion tokens needed
+            eng.submit_device(rings[i % 2], n_tb, token=False)
         M.check(L.mixer_event_record(ev1, stream))
         eng.sync()
         wall = time.perf_counter() - w0
@@ -356,7 +357,7 @@ def main():
         while time.perf_counter() < t_next:
             pass
         t_arr = time.perf_counter()
-        eng.submit_device(rings[i % 2], n_tb)
+        eng.submit_device(rings[i % 2], n_tb, token=False)
         M.check(L.mixer_read_image(eng.h, M.ptr(img), M.ptr(vmax), M.ptr(vmin), M.C.byref(idx)))
         lat.append((time.perf_counter() - t_arr) * 1e3)
         t_next = max(t_next + period, time.perf_counter())

@@ -397,7 +397,7 @@ int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkt
         x->last_arrival_ns = arrival_ns;
         if (++x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
     }
-    MIXER_TRY(new_token(x, x->s_copy, token));
+    if (token) MIXER_TRY(new_token(x, x->s_copy, token));
     { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
     return MIXER_OK;
 }
@@ -445,7 +445,7 @@ int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arriv
         MIXER_HIP(hipEventRecord(x->copy_done, x->s_copy));
         MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
     }
-    MIXER_TRY(new_token(x, x->s_comp, token));
+    if (token) MIXER_TRY(new_token(x, x->s_comp, token));
     { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
     return MIXER_OK;
 }

@@ -425,10 +425,13 @@ class Engine:
               "mixer_submit_heaps")
         return tok.value
 
-    def submit_device(self, d: DeviceBuffer, n_tb: int, arrival_ns: int = 0, offset: int = 0) -> int:
+    def submit_device(self, d: DeviceBuffer, n_tb: int, arrival_ns: int = 0, offset: int = 0,
+                      token: bool = True) -> int:
+        """token=False: no completion token (and no event record on the compute stream); 0."""
         tok = C.c_uint64(0)
         p = C.c_void_p(d.ptr.value + offset)
-        check(lib().mixer_submit_device(self.h, p, n_tb, arrival_ns, C.byref(tok)), "mixer_submit_device")
+        check(lib().mixer_submit_device(self.h, p, n_tb, arrival_ns, C.byref(tok) if token else None),
+              "mixer_submit_device")
         return tok.value
 
     def token_wait(self, tok: int):

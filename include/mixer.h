@@ -203,7 +203,10 @@ int mixer_get_cfg(mixer_ctx* ctx, mixer_array_cfg* cfg);
 int mixer_submit_heaps(mixer_ctx* ctx, const uint8_t* pkts, size_t pitch, int npkts,
                        uint64_t arrival_ns, uint64_t* token);
 /* Device-resident input: header-stripped payloads [n_tb][n_elem][payload] already in HBM
- * (the replayed-ring benchmark path). The buffer must stay valid until the token completes. */
+ * (the replayed-ring benchmark path). The buffer must stay valid until the token completes.
+ * For both submit calls token may be NULL: then no completion event is recorded (an event
+ * record idles the compute stream for a few microseconds), for callers that never reuse the
+ * buffer or order its reuse themselves (mixer_sync). */
 int mixer_submit_device(mixer_ctx* ctx, const uint8_t* d_payloads, int n_tb, uint64_t arrival_ns,
                         uint64_t* token);
 int mixer_token_done(mixer_ctx* ctx, uint64_t token); /* 1 done, 0 pending, <0 error */

@@ -262,9 +262,9 @@ def test_engine_device_submit_matches_host_submit():
     d = M.DeviceBuffer.from_array(bb)
     with M.Engine(cfg) as e:
         tb_bytes = N * 64 * C
-        # uneven device pieces: 5 + 20 + 7 blocks
-        e.submit_device(d, 5, offset=0)
-        e.submit_device(d, 20, offset=5 * tb_bytes)
+        # uneven device pieces: 5 + 20 + 7 blocks; the first two without completion tokens
+        assert e.submit_device(d, 5, offset=0, token=False) == 0
+        e.submit_device(d, 20, offset=5 * tb_bytes, token=False)
         e.token_wait(e.submit_device(d, 7, offset=25 * tb_bytes))
         e.sync()
         v, idx = e.dump()
