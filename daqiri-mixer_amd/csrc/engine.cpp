@@ -185,10 +185,14 @@ int finish_integration(mixer_ctx* x) {
                                img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng));
         MIXER_TRY(time_end(x, 3, t0));
     }
-    hipEvent_t e;
-    MIXER_TRY(take_event(x, &e));
-    MIXER_HIP(hipEventRecord(e, x->s_comp));
-    x->pending.push_back({idx, e, x->last_arrival_ns});
+    // latency probe, at most one in flight: each event record idles the compute stream for a
+    // few microseconds, and at real-time rates the previous probe has long completed
+    if (x->pending.empty()) {
+        hipEvent_t e;
+        MIXER_TRY(take_event(x, &e));
+        MIXER_HIP(hipEventRecord(e, x->s_comp));
+        x->pending.push_back({idx, e, x->last_arrival_ns});
+    }
     x->n_integ.store(idx + 1);
     x->st.integrations = idx + 1;
     x->upd_in_integ = 0;

@@ -184,7 +184,8 @@ typedef struct mixer_stats {
     uint64_t misalign;      /* submissions whose first seq % n_elem != 0 (src/rx_host_corr.cu:135) */
     uint64_t seq_gaps;      /* time blocks dropped because a seq inside them was missing */
     uint64_t wrap128;       /* corner-turn or correlator workgroups that saw an imaginary -128 */
-    double last_dump_latency_ms; /* arrival of the integration's last heap -> V/image ready */
+    double last_dump_latency_ms; /* arrival of the integration's last heap -> V/image ready, for
+                                    the latest probed integration (one probe in flight) */
 } mixer_stats;
 
 int mixer_create(const mixer_array_cfg* cfg, int device, mixer_ctx** out);
