@@ -195,10 +195,103 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// ng = 32 in registers: a 32-point transform runs on 8 lanes with 4 points each (lane r of a line
+// holds points r, r + 8, r + 16, r + 24). A 4-point DFT in registers, the twiddles W32^(r k1),
+// then an 8-point DFT across the 8 lanes in three shuffle butterflies leave lane r with
+// X[k1 + 4 bitrev3(r)], k1 = 0..3 (X[k] = sum_n x[n] W32^(nk), n = r + 8m, k = k1 + 4 k2). The
+// row and column passes meet in one LDS transpose (rows padded to 33 points), so the image takes
+// one barrier where the radix-2 Stockham passes take ten.
+__device__ __forceinline__ float2 tw32(const float2* __restrict__ tw, int q) {  // W32^q, q < 32
+    const float2 t = tw[q & 15];  // the table holds W32^0..W32^15
+    return q < 16 ? t : make_float2(-t.x, -t.y);
+}
+
+__device__ __forceinline__ void fft32_line(float2 v[4], int r, const float2* __restrict__ tw) {
+    // 4-point DFT over m, W4 = -i; (-i) (x + iy) = y - ix
+    const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+    v[1] = make_float2(d02.x + d13.y, d02.y - d13.x);
+    v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+    v[3] = make_float2(d02.x - d13.y, d02.y + d13.x);
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) v[k1] = cmul(v[k1], tw32(tw, r * k1));
+    // 8-point decimation-in-frequency DFT across the lanes r ^ h (h = 4, 2, 1): the lower lane of
+    // a pair keeps a + b, the upper (a - b) W_{2h}^(r mod h), W_{2h}^j = W32^(16 j / h)
+#pragma unroll
+    for (int h = 4; h >= 1; h >>= 1) {
+        const bool upper = (r & h) != 0;
+        const float2 w = tw32(tw, (16 / h) * (r & (h - 1)));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            const float2 p = make_float2(__shfl_xor(v[k1].x, h), __shfl_xor(v[k1].y, h));
+            v[k1] = upper ? cmul(make_float2(p.x - v[k1].x, p.y - v[k1].y), w)
+                          : make_float2(v[k1].x + p.x, v[k1].y + p.y);
+        }
+    }
+}
+
+__device__ __forceinline__ void image32_body(unsigned long long* __restrict__ G,
+                                             const float2* __restrict__ tw, float* __restrict__ img,
+                                             float* __restrict__ vmax, float* __restrict__ vmin,
+                                             int c, unsigned char* smem) {
+    constexpr int ng = 32, half = 16, TS = 33;  // transpose row stride (float2)
+    float2* T = (float2*)smem;                  // [32][33]
+    __shared__ float red_mx[4], red_mn[4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = lane & 7, line = (tid >> 6) * 8 + (lane >> 3);  // 32 lines of 8 lanes
+    const int br = ((r & 1) << 2) | (r & 2) | ((r >> 2) & 1);      // bitrev3(r)
+    unsigned long long* Gc = G + (size_t)c * 2 * ng * ng;
+    float2 v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {  // row y = line, column x = r + 8 m
+        const int q = line * ng + r + 8 * m;
+        v[m] = make_float2((float)(long long)gld(&Gc[2 * q]), (float)(long long)gld(&Gc[2 * q + 1]));
+        gst(&Gc[2 * q], 0ull);
+        gst(&Gc[2 * q + 1], 0ull);
+    }
+    fft32_line(v, r, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) T[line * TS + k1 + 4 * br] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) v[m] = T[(r + 8 * m) * TS + line];  // column kx = line
+    fft32_line(v, r, tw);
+    float mx = -1e30f, mn = 1e30f;
+    float* im = img + (size_t)c * ng * ng;
+    const int x = (line + half) & (ng - 1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {  // F[ky][kx] at ky = k1 + 4 br, fftshifted
+        const int y = (k1 + 4 * br + half) & (ng - 1);
+        const float val = v[k1].x;
+        gst(&im[y * ng + x], val);
+        mx = fmaxf(mx, val);
+        mn = fminf(mn, val);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+        mn = fminf(mn, __shfl_xor(mn, off));
+    }
+    if (lane == 0) { red_mx[tid >> 6] = mx; red_mn[tid >> 6] = mn; }
+    __syncthreads();
+    if (tid == 0) {
+        float a = red_mx[0], b = red_mn[0];
+        for (int k = 1; k < 4; ++k) { a = fmaxf(a, red_mx[k]); b = fminf(b, red_mn[k]); }
+        gst(&vmax[c], a);
+        gst(&vmin[c], b);
+    }
+}
+
 __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
                                            const float2* __restrict__ tw, float* __restrict__ img,
                                            float* __restrict__ vmax, float* __restrict__ vmin,
                                            int ng, int c, unsigned char* smem) {
+    if (ng == 32) {
+        image32_body(G, tw, img, vmax, vmin, c, smem);
+        return;
+    }
     const int nn = ng * ng;
     float2* F0 = (float2*)smem;  // [nn]
     float2* F1 = F0 + nn;        // [nn]
