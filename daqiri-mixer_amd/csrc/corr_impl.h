@@ -64,8 +64,10 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
     if constexpr (V == 3) {
         // two register sets: set s0 holds even blocks, s1 odd blocks; the loop body is written
-        // out for both parities so both sets keep fixed registers (the wait before each LDS
-        // store then covers only the older set's loads)
+        // out for both parities so both sets keep fixed registers. Every load and store is
+        // unconditional (past the end the last block is loaded and staged again, and never
+        // computed): a load skipped on some path would make the compiler's wait before the
+        // older set's LDS store cover the newer set's loads too, one block of lead instead of two.
         auto ident = [](int b) { return b; };
         using St = Stager<NE, NBLK, NTH>;
         St s0, s1;
@@ -93,21 +95,20 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
         s0.store(lds0, ident, ltid);
         __syncthreads();
         const int8_t* nx = src + 2 * m.tb;
+        const int8_t* last = src + (size_t)(n_kb - 1) * m.tb;
         int kb = 0;
         for (; kb + 2 <= n_kb; kb += 2) {
             // even step: block kb in lds0, s1 = kb+1, load kb+2 into s0
-            if (kb + 2 < n_kb) gload(s0, nx);
+            gload(s0, kb + 2 < n_kb ? nx : last);
             block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds0, lrow, kg, ar, ai, br, bi);
             if (chk) check(s1);
             s1.store(lds1, ident, ltid);
             __syncthreads();
             // odd step: block kb+1 in lds1, s0 = kb+2, load kb+3 into s1
-            if (kb + 3 < n_kb) gload(s1, nx + m.tb);
+            gload(s1, kb + 3 < n_kb ? nx + m.tb : last);
             block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds1, lrow, kg, ar, ai, br, bi);
-            if (kb + 2 < n_kb) {
-                if (chk) check(s0);
-                s0.store(lds0, ident, ltid);
-            }
+            if (chk) check(s0);
+            s0.store(lds0, ident, ltid);
             __syncthreads();
             nx += 2 * m.tb;
         }

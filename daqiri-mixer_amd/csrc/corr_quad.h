@@ -146,7 +146,9 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
         }
     } else {
         // block j in register set j & 1, both parities written out so the sets keep fixed
-        // registers (the wait before each store covers only the older set's loads)
+        // registers. Loads and stores are unconditional (past the end the last block is loaded
+        // and staged again, never computed), so the wait before each store covers only the
+        // older set's loads: a load skipped on some path makes the compiler wait for both.
         St s0, s1;
         s0.load(src, m, gblk, q4, C);
         s1.load(src + (size_t)(n_kb > 1 ? 1 : 0) * m.tb, m, gblk, q4, C);
@@ -154,19 +156,18 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
         s0.store(lds0);
         __syncthreads();
         const int8_t* nx = src + 2 * m.tb;
+        const int8_t* last = src + (size_t)(n_kb - 1) * m.tb;
         int kb = 0;
         for (; kb + 2 <= n_kb; kb += 2) {
-            if (kb + 2 < n_kb) s0.load(nx, m, gblk, q4, C);
+            s0.load(kb + 2 < n_kb ? nx : last, m, gblk, q4, C);
             compute(lds0);
             if (chk) saw |= s1.saw80();
             s1.store(lds1);
             __syncthreads();
-            if (kb + 3 < n_kb) s1.load(nx + m.tb, m, gblk, q4, C);
+            s1.load(kb + 3 < n_kb ? nx + m.tb : last, m, gblk, q4, C);
             compute(lds1);
-            if (kb + 2 < n_kb) {
-                if (chk) saw |= s0.saw80();
-                s0.store(lds0);
-            }
+            if (chk) saw |= s0.saw80();
+            s0.store(lds0);
             __syncthreads();
             nx += 2 * m.tb;
         }
