@@ -9,7 +9,8 @@
 //   (d0, -1)  4-block diagonal triangle at blocks d0..d0+3 (10 tiles per channel): wave (ch, s)
 //             runs the fold pair (s, 3 - s) of channel ch (5 tiles)
 //   (r0, c0)  rectangle rows r0, r0+1 x columns c0..c0+3 below the diagonal (8 tiles per
-//             channel): wave (ch, s) runs row s of channel ch against the 4 columns (4 tiles)
+//             channel): wave (ch, s) runs both rows of channel ch against columns c0+2s, c0+2s+1
+//             (4 tiles from 4 fragments; one row against 4 columns reads 5)
 // Per channel the items cover the lower triangle exactly once (N = 256: triangles 0-3, 4-7 and
 // rectangles rows 4-5, 6-7 x columns 0-3).
 #pragma once
@@ -235,11 +236,11 @@ void k_xcorr_quad(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
         const int r0 = it.x, c0 = it.y;  // LDS slots 0, 1: rows r0, r0 + 1; 2..5: columns c0..c0+3
         auto gblk = [=](int b) { return b < 2 ? r0 + b : c0 + b - 2; };
         if (s == 0)
-            quad_wave<V, 6, 0, 2, 6, -1, 0, 0>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
-                                               accumulate, chk, saw);
+            quad_wave<V, 6, 0, 2, 4, 1, 2, 4>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                              accumulate, chk, saw);
         else
-            quad_wave<V, 6, 1, 2, 6, -1, 0, 0>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
-                                               accumulate, chk, saw);
+            quad_wave<V, 6, 0, 4, 6, 1, 4, 6>(src, xs, gblk, q4, C, lds0, lds1, n_kb, ch, valid, a,
+                                              accumulate, chk, saw);
     }
     count_wrap(wrap, saw);
 }
