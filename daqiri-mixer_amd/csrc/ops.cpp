@@ -79,7 +79,7 @@ int plan_kind(const mixer_array_cfg& c) {
     // straight from the heaps with register staging, arrays of 4 or 8 blocks use 4-channel QUAD
     // items (256 x 128 x 8192: 0.216 vs 0.25 ms for the per-channel fold). Larger arrays stay
     // on the per-channel BLOCKED layout: their QUAD rectangles move twice the bytes through L2
-    // per tile and measured slower (1024 x 256 x 4096: 3.07 vs 2.81 ms). The direct-to-LDS
+    // per tile and measured slower (1024 x 256 x 4096: 2.91 vs 2.80 ms). The direct-to-LDS
     // variants keep the per-channel layouts.
     if (plan_fused(c) && (nb == 4 || nb == 8) && (pf == 1 || pf == 3)) return XcorrPlan::QUAD;
     return nb <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
