@@ -394,6 +394,8 @@ def test_autotune_sets_a_valid_plan():
     (256, 5, 8, 1), (256, 3, 12, 3), (128, 6, 4, 1), (128, 9, 6, 2), (32, 3, 3, 1), (64, 9, 4, 1),
     (96, 2, 2, 1), (160, 3, 5, 1),
     (224, 2, 3, 1), (384, 2, 2, 1), (512, 3, 3, 1), (1024, 1, 2, 1), (640, 1, 4, 2),
+    # odd block counts per time slot (7; 3 and 4) through the two-ahead pipelines' last pass
+    (256, 6, 7, 1), (512, 2, 7, 1),
 ])
 def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
     """The correlator staging straight from the heaps gives the CPU correlator's V, for every
