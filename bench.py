@@ -291,7 +291,7 @@ def main():
         eng.sync()
 
     # ---- throughput: K steps bracketed by events on the compute stream. Events between kernels
-    # cost the stream ~10 us each, so `value` comes from steps with no other events; the same K
+    # cost the stream about 2 us each, so `value` comes from steps with no other events; the same K
     # steps are then repeated with HIP events bracketing every launch of the dominant kernel
     # (the correlator) on that stream for its average launch time (the roofline's) ----
     ev0, ev1 = M.C.c_void_p(), M.C.c_void_p()
