@@ -365,12 +365,26 @@ int mixer_get_cfg(mixer_ctx* x, mixer_array_cfg* c) {
 void* mixer_compute_stream(mixer_ctx* x) { return x ? (void*)x->s_comp : nullptr; }
 
 // Input seam: src/rx_host_corr.cu:134-141 (alignment, groups of n_elem packets) and :159 (the
-// header-strip copy), with per-packet seq checks the reference does not make.
+// header-strip copy), with per-packet seq checks the reference does not make. Consecutive
+// valid time blocks are consecutive rows at the source pitch and consecutive ring slots, so a
+// run of them (up to kMaxCopyRows packets, within one ring buffer) goes as one 2-D copy.
+constexpr int kMaxCopyRows = 16384;
+
 int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkts,
                        uint64_t arrival_ns, uint64_t* token) {
     if (!x || !pkts || npkts < 0 || pitch < mixer_wire_bytes(&x->cfg)) return MIXER_ERR_ARG;
     MIXER_HIP(hipSetDevice(x->device));
     const int N = x->cfg.n_elem, sb = x->cfg.seq_byte, hdr = x->cfg.hdr_bytes;
+    const uint8_t* run0 = nullptr;  // first packet of the pending run of valid blocks
+    int run_n = 0;                  // its time blocks, the last run_n filled ring slots
+    auto flush = [&]() -> int {
+        if (!run_n) return MIXER_OK;
+        MIXER_HIP(hipMemcpy2DAsync(x->d_ring[x->ring_cur] + (size_t)(x->ring_fill - run_n) * x->tb_bytes,
+                                   x->payload, run0 + hdr, pitch, x->payload, (size_t)N * run_n,
+                                   hipMemcpyHostToDevice, x->s_copy));
+        run_n = 0;
+        return MIXER_OK;
+    };
     int i = 0;
     if (npkts > 0) {
         const uint32_t seq0 = read_seq_be(pkts + sb);
@@ -382,6 +396,7 @@ int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkt
         const uint8_t* g = pkts + (size_t)i * pitch;
         const uint32_t s0 = read_seq_be(g + sb);
         if (s0 % N) {  // lost alignment after a gap: resynchronise on the next block start
+            MIXER_TRY(flush());
             ++i;
             continue;
         }
@@ -390,17 +405,20 @@ int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkt
             if (read_seq_be(g + (size_t)k * pitch + sb) != s0 + (uint32_t)k) { bad = k; break; }
         if (bad >= 0) {
             { std::lock_guard<std::mutex> lk(x->mtx); x->st.seq_gaps++; }
+            MIXER_TRY(flush());
             i += bad;
             continue;
         }
-        MIXER_HIP(hipMemcpy2DAsync(x->d_ring[x->ring_cur] + (size_t)x->ring_fill * x->tb_bytes,
-                                   x->payload, g + hdr, pitch, x->payload, N,
-                                   hipMemcpyHostToDevice, x->s_copy));
+        if (run_n == 0) run0 = g;
+        ++run_n;
         x->st.heaps_in += N;
         i += N;
         x->last_arrival_ns = arrival_ns;
-        if (++x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
+        ++x->ring_fill;
+        if (x->ring_fill == x->n_tb_upd || N * (run_n + 1) > kMaxCopyRows) MIXER_TRY(flush());
+        if (x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
     }
+    MIXER_TRY(flush());
     if (token) MIXER_TRY(new_token(x, x->s_copy, token));
     { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
     return MIXER_OK;
