@@ -252,6 +252,30 @@ def test_engine_misalign_and_gap():
     assert np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
 
 
+def test_engine_gap_splits_a_copy_run():
+    """One call with a packet missing in block 2 of 9: blocks 0, 1 | 3, 4 share a ring buffer
+    (the run before the gap is copied, the run after it starts at the next slot), and the run
+    3.. crosses into the next ring buffer. Both integrations are exact."""
+    N, C = 64, 4
+    cfg = cfg_for(N, C, 128, flags=M.CFG_CUMULATIVE)
+    pitch = 640
+    pk = M.synth_packets(cfg, M.default_sky(sigma=0.5), seed=9, seq0=0, npkts=9 * N, pitch=pitch)
+    keep = np.ones(len(pk), bool)
+    keep[2 * N + 17] = False
+    with M.Engine(cfg) as e:
+        e.token_wait(e.submit_heaps(np.ascontiguousarray(pk[keep]), pitch, int(keep.sum())))
+        e.sync()
+        st = e.stats()
+        v, idx = e.dump()
+        vc, _ = e.dump(cumulative=True)
+    assert st.seq_gaps == 1 and st.time_blocks == 8 and idx == 2
+    bb = heaps_of(pk, N, C)
+    r1, i1 = O.v_full_from_heaps(np.ascontiguousarray(bb[[0, 1, 3, 4]]), 4, N, C, 32)
+    r2, i2 = O.v_full_from_heaps(np.ascontiguousarray(bb[5:9]), 4, N, C, 32)
+    assert np.array_equal(v, np.stack([r2, i2], 1).astype(np.float32))
+    assert np.array_equal(vc, np.stack([r1, i1], 1).astype(np.float32) + np.stack([r2, i2], 1).astype(np.float32))
+
+
 def test_engine_device_submit_matches_host_submit():
     N, C = 256, 16
     cfg = cfg_for(N, C, 512)
