@@ -197,7 +197,8 @@ int mixer_get_cfg(mixer_ctx* ctx, mixer_array_cfg* cfg);
  * multiple of n_elem and the first seq a multiple of n_elem; otherwise the leading partial
  * group is skipped and misalign is counted, as the reference's drain thread does. Packets
  * are checked for consecutive seq; a time block with a gap is dropped (seq_gaps). The header
- * strip is a hipMemcpy2DAsync on the context's copy stream. *token receives a completion
+ * strip is one hipMemcpy2DAsync per run of consecutive valid time blocks (at most 16384
+ * packets, within one ring buffer) on the context's copy stream. *token receives a completion
  * token: the caller's buffer is free for reuse once mixer_token_done/wait reports it (the
  * reference returns its packets to the NIC before the copy has run; this API does not).
  * arrival_ns (steady clock) of the last packet feeds the dump-latency statistics. */
