@@ -1,6 +1,6 @@
 // corr_impl.h -- the correlator: lower-triangle complex int8 GEMM V += X X^H on
 // v_mfma_i32_32x32x32_i8 with int32 accumulation into a tile-packed accumulator (replaces
-// cgemm_blk_t<ACC>, src/cgemm.cuh:84-125).
+// cgemm_blk_t<ACC>, src/cgemm.cuh:28-69).
 //   k_xcorr_fold<NB,V>  arrays of NB <= 8 blocks of 32 elements (N <= 256)
 //   k_xcorr_blk<NB,V>   arrays of NB = 12..32 blocks (N = 384..1024, N % 128 == 0)
 // V selects the time-loop pipeline (the plan's "prefetch" field):

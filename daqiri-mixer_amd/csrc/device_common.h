@@ -14,7 +14,7 @@ namespace mixer {
 
 
 // Per-byte two's-complement negation of 4 packed int8 (wraps -128 to -128 like the
-// reference's (int8)(-im), src/rx_host_corr.cu:71 and src/cgemm.cuh:108).
+// reference's (int8)(-im), src/rx_host_corr.cu:71 and src/cgemm.cuh:52).
 __device__ __forceinline__ int neg8x4(int w) {
     unsigned nw = ~(unsigned)w;
     return (int)(((nw & 0x7f7f7f7fu) + 0x01010101u) ^ (nw & 0x80808080u));
@@ -144,7 +144,7 @@ __device__ __forceinline__ void store_tile(const v16i& x, int32_t* __restrict__ 
 }
 
 // The four MFMAs of one tile and one 32-sample step, in the stacked real form of the
-// reference's four real products (cgemm.cuh:115-118):
+// reference's four real products (cgemm.cuh:59-62):
 //   Re += re_I . re_J + wrap(-im_I) . wrap(-im_J)      Im += re_I . wrap(-im_J) + im_I . re_J
 struct Frag { v4i re, im, ng; };
 __device__ __forceinline__ void tile_step(const Frag& a, const Frag& b, v16i& vr, v16i& vi) {

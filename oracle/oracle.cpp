@@ -71,9 +71,9 @@ static inline int32_t sat_count(int64_t v, long* of) {
     return (int32_t)(uint32_t)(uint64_t)v;
 }
 
-/* src/cgemm.cuh:84-125: per k, aneg = -aim narrowed to int8 (cgemm.cuh:108), then
- * cre += are*bre + aneg*bim, cim += are*bim + aim*bre (cgemm.cuh:115-118). B is [N][K] read
- * column-major, so B[k][n] = b[n*K + k] (cgemm.cuh:111). */
+/* src/cgemm.cuh:28-69: per k, aneg = -aim narrowed to int8 (cgemm.cuh:52), then
+ * cre += are*bre + aneg*bim, cim += are*bim + aim*bre (cgemm.cuh:59-62). B is [N][K] read
+ * column-major, so B[k][n] = b[n*K + k] (cgemm.cuh:55). */
 long orc_cgemm(int M, int N, int K, int batch,
                const int8_t* are, const int8_t* aim, const int8_t* bre, const int8_t* bim,
                int32_t* cre, int32_t* cim, int acc) {

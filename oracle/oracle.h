@@ -30,10 +30,10 @@ uint32_t orc_read_seq(const uint8_t* pkt, int seq_byte);
 void orc_unpack(const uint8_t* bb, int n_tb, int N, int C, int T,
                 int8_t* are, int8_t* aim, int8_t* bre, int8_t* bim);
 
-/* ---- a4: cgemm_blk_t<ACC> (src/cgemm.cuh:84-125) -------------------------------------------- */
+/* ---- a4: cgemm_blk_t<ACC> (src/cgemm.cuh:28-69) -------------------------------------------- */
 /* Per batch b: Cre += Are*Bre + wrap8(-Aim)*Bim ; Cim += Are*Bim + Aim*Bre, with A [M][K]
- * row-major and B [N][K] (the reference reads its [N][K] B column-major, cgemm.cuh:111).
- * acc=0 overwrites C, acc=1 adds into it (cgemm.cuh:98-101). Sums are formed in int64; the
+ * row-major and B [N][K] (the reference reads its [N][K] B column-major, cgemm.cuh:55).
+ * acc=0 overwrites C, acc=1 adds into it (cgemm.cuh:42-45). Sums are formed in int64; the
  * return value is the number of outputs that do not fit int32 (0 = exact). Full square. */
 long orc_cgemm(int M, int N, int K, int batch,
                const int8_t* are, const int8_t* aim, const int8_t* bre, const int8_t* bim,

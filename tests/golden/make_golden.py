@@ -5,7 +5,7 @@
 Case: 64 elements on an 8x8 grid, 3 channels, 2 time blocks (64 samples), complex int8 in
 [-127, 127] plus real-part -128 samples (which do not wrap), 32x32 image.
   heaps      uint8 [2][64][2*3*32]    header-stripped config.h payloads (planar re | im)
-  vre, vim   int64 [3][64][64]        V = X X^H (src/cgemm.cuh:115-118 for this domain)
+  vre, vim   int64 [3][64][64]        V = X X^H (src/cgemm.cuh:59-62 for this domain)
   v_f32      f32   [3][2][64][64]     the --dump record of one integration
   grid       c128  [3][32][32]        sum of V over baselines with (qa-qb, pa-pb) mod 32
   img        f32   [3][32][32]        Re of the fftshifted forward unnormalised 2-D DFT

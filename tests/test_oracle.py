@@ -3,7 +3,7 @@
 Each check is chosen so that a plausible mistake in the restatement (a dropped term, a wrong
 sign, a transposed operand, an off-by-one cell) fails: numpy's complex128 matmul and FFT are
 independent formulations, the k_sky signal gives rank-1 closed forms, and the -128 wrap case is
-computed by hand from src/cgemm.cuh:108,115-118 and src/rx_host_corr.cu:71.
+computed by hand from src/cgemm.cuh:52,59-62 and src/rx_host_corr.cu:71.
 """
 import numpy as np
 import pytest
@@ -96,7 +96,7 @@ def test_cgemm_accumulate_is_sum_of_updates():
 
 
 def test_cgemm_minus128_wrap_by_hand():
-    """x_a = (0, -128), x_b = (0, 1), one sample. cgemm.cuh:108 narrows -aim to int8 and
+    """x_a = (0, -128), x_b = (0, 1), one sample. cgemm.cuh:52 narrows -aim to int8 and
     rx_host_corr.cu:71 stores Bim = (int8)(-im), so wrap8(-(-128)) = -128:
       Vr[a][b] = re_a re_b + wrap(-im_a) wrap(-im_b) = 0 + (-128)(-1) = 128   (true value -128)
       Vr[a][a] = (-128)(-128) = 16384
