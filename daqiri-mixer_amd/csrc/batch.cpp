@@ -20,6 +20,7 @@ struct mixer_batch {
     std::vector<BatchJobDev> jobs;
     std::vector<int> slot_cap;  // accumulator capacity (time slots) per job
     BatchJobDev* d_jobs = nullptr;
+    bool jobs_dirty = false;    // host job table changed but not yet uploaded to d_jobs
     int4* d_items = nullptr;
     int items_cap = 0, n_items = 0, n_wgs = 0;
     int* d_counter = nullptr;  // one per XCD queue: [0, 8) fold items, [8, 16) BLOCKED items
@@ -281,16 +282,21 @@ int mixer_batch_destroy(mixer_batch* b) {
 int mixer_batch_run(mixer_batch* b, const uint8_t* const* d_heaps) {
     if (!b || !d_heaps) return MIXER_ERR_ARG;
     MIXER_HIP(hipSetDevice(b->device));
-    bool changed = false;
-    for (size_t j = 0; j < b->jobs.size(); ++j) {
+    // validate every pointer before touching the job table, so a refused call leaves the host
+    // table equal to the device copy
+    for (size_t j = 0; j < b->jobs.size(); ++j)
         if (!d_heaps[j]) return MIXER_ERR_ARG;
+    bool changed = b->jobs_dirty;
+    for (size_t j = 0; j < b->jobs.size(); ++j)
         if (b->jobs[j].heaps != d_heaps[j]) { b->jobs[j].heaps = d_heaps[j]; changed = true; }
-    }
     if (changed) {
-        // the table is read by the graph's kernels at run time; update it in stream order
+        // the table is read by the graph's kernels at run time; update it in stream order. It
+        // stays dirty until the upload has succeeded.
+        b->jobs_dirty = true;
         MIXER_HIP(hipStreamSynchronize(b->s));
         MIXER_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJobDev) * b->jobs.size(),
                             hipMemcpyHostToDevice));
+        b->jobs_dirty = false;
     }
     b->have_inputs = true;
     if (b->exec) return hip_status(hipGraphLaunch(b->exec, b->s));
@@ -347,6 +353,9 @@ int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots
     return MIXER_OK;
 }
 
+// Times every candidate as the median of reps individually timed runs (after one warm-up run)
+// and moves off the starting tuning only for a candidate at least 2 % faster than the best so far,
+// so run-to-run noise cannot make the tuned batch slower than the one it started from.
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best) {
     if (!b || !best || reps <= 0) return MIXER_ERR_ARG;
     if (!b->have_inputs) return MIXER_ERR_STATE;
@@ -355,26 +364,38 @@ int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best) {
     MIXER_HIP(hipEventCreate(&e0));
     MIXER_HIP(hipEventCreate(&e1));
     const mixer_batch_tuning start = b->tun;
+    auto median_ms = [&](float* med) -> int {
+        MIXER_TRY(plan(b));
+        auto run = [&]() { return b->exec ? hip_status(hipGraphLaunch(b->exec, b->s)) : enqueue(b); };
+        MIXER_TRY(run());  // warm-up
+        std::vector<float> t;
+        for (int r = 0; r < reps; ++r) {
+            MIXER_HIP(hipEventRecord(e0, b->s));
+            MIXER_TRY(run());
+            MIXER_HIP(hipEventRecord(e1, b->s));
+            MIXER_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            MIXER_HIP(hipEventElapsedTime(&ms, e0, e1));
+            t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        *med = t.size() % 2 ? t[t.size() / 2] : 0.5f * (t[t.size() / 2 - 1] + t[t.size() / 2]);
+        return MIXER_OK;
+    };
     mixer_batch_tuning win = start;
-    float best_ms = 1e30f;
-    int st = MIXER_OK;
+    float best_ms = 0;
+    int st = median_ms(&best_ms);
     for (int pack : {0, 1}) {
         for (int tbt : {64, 128, 256, 512}) {
+            if (st) break;
+            if (pack == start.pack && tbt == start.tb_target) continue;  // measured first
             b->tun.pack = pack;
             b->tun.tb_target = tbt;
             b->tun.use_graph = start.use_graph;
-            if ((st = plan(b))) break;
-            auto run = [&]() { return b->exec ? hip_status(hipGraphLaunch(b->exec, b->s)) : enqueue(b); };
-            if ((st = run())) break;
-            if ((st = hip_status(hipEventRecord(e0, b->s)))) break;
-            for (int r = 0; r < reps && !st; ++r) st = run();
-            if (st || (st = hip_status(hipEventRecord(e1, b->s)))) break;
-            if ((st = hip_status(hipEventSynchronize(e1)))) break;
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best_ms) { best_ms = ms; win = b->tun; }
+            if ((st = median_ms(&ms))) break;
+            if (ms < 0.98f * best_ms) { best_ms = ms; win = b->tun; }
         }
-        if (st) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

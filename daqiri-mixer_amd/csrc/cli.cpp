@@ -4,15 +4,21 @@
 //   mixer_corr [cfg|-] [--seconds N] [--fps F] [--device D] [--noflop] [--flush N] [--dump PATH]
 //              [--dump-mode cumulative|integ] [--elements N] [--grid-x G] [--channels C]
 //              [--k-update K] [--ng NG] [--rate GBps] [--sigma S] [--seed X] [--shm NAME]
-//              [--rho R] [--torbit S] [--refresh-hz F] [--eth-dst MAC]
+//              [--rho R] [--torbit S] [--refresh-hz F] [--eth-dst MAC] [--frame dv|latest]
+//              [--slots N]
 // --rho/--torbit/--refresh-hz move the synthetic planet on the orbit (l, m) = rho (cos, sin)
 // (2 pi t / torbit), re-generating one replay slot per refresh (src/tx_host.cu:84-86,116-125);
 // --eth-dst sets the wire header's destination MAC (src/tx_host.cu:87,96-98).
+// --frame dv (default) publishes the image of everything integrated since the previous frame,
+// as the reference's monitor does (dV, src/rx_host_corr.cu:218-222); --frame latest publishes the
+// latest integration's image. --slots N bounds the ingest slots (default 96, the reference's
+// NSLOT, src/rx_host_corr.cu:46): a time block that finds none free is dropped (imissed).
 //
 // The first positional argument (the reference's DAQIRI YAML) is accepted and ignored.
 // --flush N sets the samples per integration to N V-updates (the reference flushes its int32
 // cube to f32 every N V-updates). --dump appends f32 [C][2][N][N] records about once per
-// second, the cumulative-since-start cube by default as the reference does. The image cube is
+// second, the cumulative-since-start cube by default as the reference does, and writes a JSON
+// sidecar PATH.json describing the records (the reference's records are headerless). The image cube is
 // published to /dev/shm/corr32 (Corr32 layout) at --fps. Any library error exits with 1.
 #include <fcntl.h>
 #include <sys/mman.h>
@@ -54,7 +60,8 @@ int main(int argc, char** argv) {
     cfg.k_update = 1024;  // KPCH (src/rx_host_corr.cu:40)
     int seconds = 0, device = 0, flush = 8;
     float fps = 20.f;
-    bool noflop = false, dump_cum = true;
+    bool noflop = false, dump_cum = true, frame_dv = true;
+    int slots = 96;
     const char* dump_path = nullptr;
     const char* shm_name = "/corr32";
     double rate = 0, sigma = 0.5, rho = 0.525, torbit = 3.0, refresh_hz = 0.0;
@@ -86,6 +93,8 @@ int main(int argc, char** argv) {
         else if (a == "--rho") rho = atof(nxt());
         else if (a == "--torbit") torbit = atof(nxt());
         else if (a == "--refresh-hz") refresh_hz = atof(nxt());
+        else if (a == "--frame") frame_dv = std::string(nxt()) != "latest";
+        else if (a == "--slots") slots = atoi(nxt());
         else if (a == "--eth-dst") {
             const char* s = nxt();
             unsigned m[6];
@@ -99,7 +108,8 @@ int main(int argc, char** argv) {
             fprintf(stderr, "usage: %s [cfg] [--seconds N][--fps F][--device D][--noflop][--flush N][--dump PATH]\n"
                             "  [--dump-mode cumulative|integ][--elements N][--grid-x G][--channels C][--k-update K]\n"
                             "  [--ng NG][--rate GBps][--sigma S][--seed X][--shm NAME]\n"
-                            "  [--rho R][--torbit S][--refresh-hz F][--eth-dst MAC]\n", argv[0]);
+                            "  [--rho R][--torbit S][--refresh-hz F][--eth-dst MAC][--frame dv|latest][--slots N]\n",
+                    argv[0]);
             return 1;
         } else if (a[0] == '-' && a != "-") { fprintf(stderr, "unknown flag %s\n", a.c_str()); return 1; }
     }
@@ -108,12 +118,15 @@ int main(int argc, char** argv) {
     cfg.k_integ = flush * cfg.k_update;
     if (dump_path && dump_cum) cfg.flags |= MIXER_CFG_CUMULATIVE;
     if (noflop) cfg.flags |= MIXER_CFG_NOFLOP;
+    if (frame_dv) cfg.flags |= MIXER_CFG_FRAME_DV;
+    if (slots < 0) { fprintf(stderr, "--slots must be >= 0\n"); return 1; }
     CK(mixer_validate_cfg(&cfg));
     std::signal(SIGINT, on_sig);
     std::signal(SIGTERM, on_sig);
     CK(mixer_set_device(device));
     mixer_ctx* ctx = nullptr;
     CK(mixer_create(&cfg, device, &ctx));
+    CK(mixer_set_ingest_slots(ctx, slots));
 
     // Corr32 in shared memory (src/rx_host_corr.cu:100-105)
     const size_t cbytes = mixer_corr32_bytes(&cfg);
@@ -156,10 +169,8 @@ int main(int argc, char** argv) {
             uint8_t* base = ring + (size_t)slot * per_slot * N * pitch;
             if (refresh_hz > 0 && torbit > 0 && now_ns() >= next_refresh) {
                 // the planet moved: re-generate this slot at its new position (fresh noise too)
-                const double t = (now_ns() - t0) * 1e-9, w = 2.0 * M_PI / torbit;
-                mixer_sky s2 = sky;
-                s2.planet_l = rho * std::cos(w * t);
-                s2.planet_m = rho * std::sin(w * t);
+                mixer_sky s2;
+                CK(mixer_orbit_sky(&sky, rho, torbit, (now_ns() - t0) * 1e-9, &s2));
                 CK(mixer_synth_packets(&cfg, &s2, seed, seq, per_slot * N, base, pitch, 0));
                 next_refresh += (uint64_t)(1e9 / refresh_hz);
             }
@@ -185,6 +196,17 @@ int main(int argc, char** argv) {
         dumpf = fopen(dump_path, "wb");
         if (!dumpf) { perror("dump fopen"); g_stop = 1; src.join(); return 1; }
         hv.resize((size_t)cfg.n_chan * 2 * N * N);
+        const std::string side = std::string(dump_path) + ".json";
+        FILE* jf = fopen(side.c_str(), "w");
+        if (!jf) { perror("dump sidecar fopen"); g_stop = 1; src.join(); return 1; }
+        fprintf(jf,
+                "{\"format\": \"mixer-dump\", \"dtype\": \"float32-le\", "
+                "\"layout\": \"[n_chan][2][n_elem][n_elem] (plane 0 = Re, 1 = Im; row a, column b)\", "
+                "\"n_chan\": %d, \"n_elem\": %d, \"grid_x\": %d, \"record_bytes\": %zu, "
+                "\"mode\": \"%s\", \"samples_per_integration\": %d, \"records_per_second\": 1}\n",
+                cfg.n_chan, N, cfg.grid_x, hv.size() * sizeof(float),
+                dump_cum ? "cumulative" : "integration", cfg.k_integ);
+        fclose(jf);
     }
     const auto t0 = std::chrono::steady_clock::now();
     auto next = t0;

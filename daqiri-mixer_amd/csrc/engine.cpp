@@ -68,6 +68,7 @@ struct mixer_ctx {
     hipEvent_t ring_free[2] = {nullptr, nullptr};
     bool ring_used[2] = {false, false};
     int ring_cur = 0, ring_fill = 0;
+    bool copy_wait = false;  // s_copy must wait for ring_free[ring_cur] before its next copy
     int8_t* d_tiles = nullptr;
     int32_t* d_acc = nullptr;
     uint32_t* d_wrap = nullptr;
@@ -77,6 +78,11 @@ struct mixer_ctx {
     float* d_img[2] = {nullptr, nullptr};  // [C][ng][ng] then vmax[C], vmin[C]
     hipEvent_t integ_ev[2] = {nullptr, nullptr};
     hipEvent_t pub_done[2] = {nullptr, nullptr};
+    // frame (MIXER_CFG_FRAME_DV): int64 grid summed since the last mixer_read_frame, its image
+    unsigned long long* d_Gf = nullptr;
+    float* d_imgf = nullptr;
+    hipEvent_t frame_ev = nullptr, pubf_done = nullptr;
+    uint64_t frame_mark = 0;  // integrations covered by frames read so far
     hipEvent_t pub_done_cum = nullptr;
     hipEvent_t copy_done = nullptr;
     const float2* d_tw = nullptr;
@@ -85,23 +91,35 @@ struct mixer_ctx {
     std::atomic<uint64_t> n_integ{0};  // integrations enqueued
     uint64_t last_arrival_ns = 0;
 
-    // tokens: events recorded after the last use of a caller's buffer
+    // tokens: events recorded after the last use of a caller's buffer. Token t uses slot
+    // t % NTOK; a slot is re-recorded only after its previous token's event has completed, so a
+    // token whose slot holds a later token is complete.
     static constexpr int NTOK = 256;
     hipEvent_t tok_ev[NTOK] = {};
-    uint64_t tok_next = 1;  // token t uses tok_ev[t % NTOK]
+    uint64_t tok_owner[NTOK] = {};
+    uint64_t tok_next = 1;
+
+    // bounded ingest (mixer_set_ingest_slots): a time block of mixer_submit_heaps holds a slot
+    // until its header-strip copy has completed; batches of slots with their copy events
+    int slots_max = 0;  // 0: unbounded
+    int slots_busy = 0;
+    struct SlotBatch { hipEvent_t ev; int n; };
+    std::deque<SlotBatch> slot_inflight;
+    std::vector<hipEvent_t> slot_pool;  // submitting thread only
 
     // latency: per enqueued integration, (event, arrival)
     struct Pending { uint64_t idx; hipEvent_t ev; uint64_t arrival; };
     std::deque<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
 
-    // timing
+    // timing (guarded by tmtx: the submitting thread appends, mixer_get_timing resolves)
+    std::mutex tmtx;
     unsigned timing = 0;  // bit s: time stage s
     std::vector<TimedLaunch> timed;
     double t_ms[4] = {0, 0, 0, 0};
     uint64_t t_n[4] = {0, 0, 0, 0};
 
-    std::mutex mtx;  // guards the integration/dump handshake and stats
+    std::mutex mtx;  // guards the integration/dump handshake and every field of st
     mixer_stats st{};
     uint64_t frames_published = 0;
 };
@@ -119,7 +137,7 @@ int take_event(mixer_ctx* x, hipEvent_t* e) {
     return hip_status(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
 }
 
-// Resolve completed timing pairs (all of them if wait).
+// Resolve completed timing pairs (all of them if wait). Caller holds tmtx.
 int resolve_timing(mixer_ctx* x, bool wait) {
     size_t k = 0;
     for (; k < x->timed.size(); ++k) {
@@ -138,12 +156,14 @@ int resolve_timing(mixer_ctx* x, bool wait) {
 }
 
 int time_begin(mixer_ctx* x, hipEvent_t* a, int stage) {
+    std::lock_guard<std::mutex> lk(x->tmtx);
     if (!(x->timing & (1u << stage))) return MIXER_OK;
     MIXER_HIP(hipEventCreate(a));
     return hip_status(hipEventRecord(*a, x->s_comp));
 }
 int time_end(mixer_ctx* x, int stage, hipEvent_t a) {
-    if (!(x->timing & (1u << stage))) return MIXER_OK;
+    std::lock_guard<std::mutex> lk(x->tmtx);
+    if (!(x->timing & (1u << stage)) || !a) return MIXER_OK;
     hipEvent_t b;
     MIXER_HIP(hipEventCreate(&b));
     MIXER_HIP(hipEventRecord(b, x->s_comp));
@@ -182,7 +202,8 @@ int finish_integration(mixer_ctx* x) {
         MIXER_TRY(time_begin(x, &t0, 3));
         float* img = x->d_img[b];
         MIXER_HIP(launch_image(x->s_comp, x->d_G, x->d_tw, img, img + x->img_floats,
-                               img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng));
+                               img + x->img_floats + x->cfg.n_chan, x->cfg.n_chan, x->cfg.ng,
+                               x->d_Gf));
         MIXER_TRY(time_end(x, 3, t0));
     }
     // latency probe, at most one in flight: each event record idles the compute stream for a
@@ -220,8 +241,11 @@ int run_gemm(mixer_ctx* x, const uint8_t* src) {
                            x->cfg.n_chan, x->n_tb_upd, x->upd_in_integ > 0 ? 1 : 0));
     MIXER_TRY(time_end(x, 1, t0));
     x->upd_in_integ++;
-    x->st.vupdates++;
-    x->st.time_blocks += x->n_tb_upd;
+    {
+        std::lock_guard<std::mutex> lk(x->mtx);
+        x->st.vupdates++;
+        x->st.time_blocks += x->n_tb_upd;
+    }
     if (x->upd_in_integ == x->upd_per_integ) MIXER_TRY(finish_integration(x));
     return MIXER_OK;
 }
@@ -237,20 +261,49 @@ int dispatch_ring(mixer_ctx* x) {
     MIXER_HIP(hipEventRecord(x->ring_free[b], x->s_comp));
     x->ring_used[b] = true;
     if (flop && !x->plan.fused) MIXER_TRY(run_gemm(x, x->d_ring[b]));
-    if (!flop) x->st.time_blocks += x->n_tb_upd;
+    if (!flop) {
+        std::lock_guard<std::mutex> lk(x->mtx);
+        x->st.time_blocks += x->n_tb_upd;
+    }
     x->ring_cur ^= 1;
     x->ring_fill = 0;
-    if (x->ring_used[x->ring_cur]) MIXER_HIP(hipStreamWaitEvent(x->s_copy, x->ring_free[x->ring_cur], 0));
+    // s_copy waits for the compute that still reads the next ring buffer just before its next
+    // copy into it, not here: an event recorded on s_copy meanwhile (a caller's token, a slot
+    // batch) then covers the copies already enqueued and not that compute
+    x->copy_wait = x->ring_used[x->ring_cur];
     return MIXER_OK;
 }
 
+// Called before every copy into the current ring buffer.
+int ring_copy_wait(mixer_ctx* x) {
+    if (!x->copy_wait) return MIXER_OK;
+    x->copy_wait = false;
+    return hip_status(hipStreamWaitEvent(x->s_copy, x->ring_free[x->ring_cur], 0));
+}
+
 int new_token(mixer_ctx* x, hipStream_t s, uint64_t* token) {
-    const uint64_t t = x->tok_next++;
-    hipEvent_t& e = x->tok_ev[t % mixer_ctx::NTOK];
+    const uint64_t t = x->tok_next;
+    const int k = (int)(t % mixer_ctx::NTOK);
+    hipEvent_t& e = x->tok_ev[k];
     if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+    // the slot's previous token must be complete before its event is re-recorded (back-pressure
+    // when more than NTOK tokens are outstanding)
+    else MIXER_HIP(hipEventSynchronize(e));
     MIXER_HIP(hipEventRecord(e, s));
+    x->tok_owner[k] = t;
+    x->tok_next = t + 1;
     if (token) *token = t;
     return MIXER_OK;
+}
+
+// Return the slots whose header-strip copies have completed (the reference's gpu_thread polls
+// its in-flight slots the same way before taking the next one, src/rx_host_corr.cu:154).
+void retire_slots(mixer_ctx* x) {
+    while (!x->slot_inflight.empty() && hipEventQuery(x->slot_inflight.front().ev) == hipSuccess) {
+        x->slots_busy -= x->slot_inflight.front().n;
+        x->slot_pool.push_back(x->slot_inflight.front().ev);
+        x->slot_inflight.pop_front();
+    }
 }
 
 void destroy_ctx(mixer_ctx* x) {
@@ -259,10 +312,15 @@ void destroy_ctx(mixer_ctx* x) {
     if (x->s_comp) (void)hipStreamSynchronize(x->s_comp);
     if (x->s_copy) (void)hipStreamSynchronize(x->s_copy);
     if (x->s_pub) (void)hipStreamSynchronize(x->s_pub);
-    resolve_timing(x, true);
+    {
+        std::lock_guard<std::mutex> lk(x->tmtx);
+        resolve_timing(x, true);
+    }
     for (auto& p : x->pending) (void)hipEventDestroy(p.ev);
     for (auto e : x->ev_pool) (void)hipEventDestroy(e);
     for (auto e : x->tok_ev) if (e) (void)hipEventDestroy(e);
+    for (auto& sb : x->slot_inflight) (void)hipEventDestroy(sb.ev);
+    for (auto e : x->slot_pool) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b) {
         if (x->d_ring[b]) (void)hipFree(x->d_ring[b]);
         if (x->d_v[b]) (void)hipFree(x->d_v[b]);
@@ -272,6 +330,10 @@ void destroy_ctx(mixer_ctx* x) {
         if (x->pub_done[b]) (void)hipEventDestroy(x->pub_done[b]);
     }
     if (x->pub_done_cum) (void)hipEventDestroy(x->pub_done_cum);
+    if (x->frame_ev) (void)hipEventDestroy(x->frame_ev);
+    if (x->pubf_done) (void)hipEventDestroy(x->pubf_done);
+    if (x->d_Gf) (void)hipFree(x->d_Gf);
+    if (x->d_imgf) (void)hipFree(x->d_imgf);
     if (x->copy_done) (void)hipEventDestroy(x->copy_done);
     if (x->d_vcum) (void)hipFree(x->d_vcum);
     if (x->d_tiles) (void)hipFree(x->d_tiles);
@@ -328,6 +390,14 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
     const size_t gbytes = (size_t)cfg->n_chan * cfg->ng * cfg->ng * 16;
     MIXER_HIP(hipMalloc(&x->d_G, gbytes));
     MIXER_HIP(hipMemset(x->d_G, 0, gbytes));
+    if ((cfg->flags & MIXER_CFG_FRAME_DV) && !(cfg->flags & MIXER_CFG_NO_IMAGE)) {
+        MIXER_HIP(hipMalloc(&x->d_Gf, gbytes));
+        MIXER_HIP(hipMemset(x->d_Gf, 0, gbytes));
+        MIXER_HIP(hipMalloc(&x->d_imgf, (x->img_floats + 2 * cfg->n_chan) * sizeof(float)));
+        MIXER_HIP(hipEventCreateWithFlags(&x->frame_ev, hipEventDisableTiming));
+        MIXER_HIP(hipEventCreateWithFlags(&x->pubf_done, hipEventDisableTiming));
+        MIXER_HIP(hipEventRecord(x->pubf_done, x->s_pub));
+    }
     MIXER_TRY(get_twiddles(cfg->ng, &x->d_tw));
     MIXER_HIP(hipDeviceSynchronize());
     return MIXER_OK;
@@ -375,8 +445,11 @@ int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkt
     if (!x || !pkts || npkts < 0 || pitch < mixer_wire_bytes(&x->cfg)) return MIXER_ERR_ARG;
     MIXER_HIP(hipSetDevice(x->device));
     const int N = x->cfg.n_elem, sb = x->cfg.seq_byte, hdr = x->cfg.hdr_bytes;
+    retire_slots(x);
     const uint8_t* run0 = nullptr;  // first packet of the pending run of valid blocks
     int run_n = 0;                  // its time blocks, the last run_n filled ring slots
+    uint64_t d_heaps = 0, d_missed = 0, d_gaps = 0, d_misalign = 0;  // stats of this call
+    int slots_taken = 0;
     auto flush = [&]() -> int {
         if (!run_n) return MIXER_OK;
         MIXER_HIP(hipMemcpy2DAsync(x->d_ring[x->ring_cur] + (size_t)(x->ring_fill - run_n) * x->tb_bytes,
@@ -385,43 +458,73 @@ int mixer_submit_heaps(mixer_ctx* x, const uint8_t* pkts, size_t pitch, int npkt
         run_n = 0;
         return MIXER_OK;
     };
-    int i = 0;
-    if (npkts > 0) {
-        const uint32_t seq0 = read_seq_be(pkts + sb);
-        const int off = (int)((N - seq0 % N) % N);
-        if (off) { std::lock_guard<std::mutex> lk(x->mtx); x->st.misalign++; }
-        i = off;
-    }
-    while (i + N <= npkts) {
-        const uint8_t* g = pkts + (size_t)i * pitch;
-        const uint32_t s0 = read_seq_be(g + sb);
-        if (s0 % N) {  // lost alignment after a gap: resynchronise on the next block start
-            MIXER_TRY(flush());
-            ++i;
-            continue;
+    auto body = [&]() -> int {
+        int i = 0;
+        if (npkts > 0) {
+            const uint32_t seq0 = read_seq_be(pkts + sb);
+            const int off = (int)((N - seq0 % N) % N);
+            if (off) d_misalign++;
+            i = off;
         }
-        int bad = -1;
-        for (int k = 1; k < N; ++k)
-            if (read_seq_be(g + (size_t)k * pitch + sb) != s0 + (uint32_t)k) { bad = k; break; }
-        if (bad >= 0) {
-            { std::lock_guard<std::mutex> lk(x->mtx); x->st.seq_gaps++; }
-            MIXER_TRY(flush());
-            i += bad;
-            continue;
+        while (i + N <= npkts) {
+            const uint8_t* g = pkts + (size_t)i * pitch;
+            const uint32_t s0 = read_seq_be(g + sb);
+            if (s0 % N) {  // lost alignment after a gap: resynchronise on the next block start
+                MIXER_TRY(flush());
+                ++i;
+                continue;
+            }
+            int bad = -1;
+            for (int k = 1; k < N; ++k)
+                if (read_seq_be(g + (size_t)k * pitch + sb) != s0 + (uint32_t)k) { bad = k; break; }
+            if (bad >= 0) {
+                d_gaps++;
+                MIXER_TRY(flush());
+                i += bad;
+                continue;
+            }
+            // no free slot: the block is dropped and counted (src/rx_host_corr.cu:139-140)
+            if (x->slots_max > 0 && x->slots_busy >= x->slots_max) {
+                d_missed++;
+                MIXER_TRY(flush());
+                i += N;
+                continue;
+            }
+            if (x->slots_max > 0) { x->slots_busy++; slots_taken++; }
+            if (run_n == 0) {
+                MIXER_TRY(ring_copy_wait(x));
+                run0 = g;
+            }
+            ++run_n;
+            d_heaps += N;
+            i += N;
+            x->last_arrival_ns = arrival_ns;
+            ++x->ring_fill;
+            if (x->ring_fill == x->n_tb_upd || N * (run_n + 1) > kMaxCopyRows) MIXER_TRY(flush());
+            if (x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
         }
-        if (run_n == 0) run0 = g;
-        ++run_n;
-        x->st.heaps_in += N;
-        i += N;
-        x->last_arrival_ns = arrival_ns;
-        ++x->ring_fill;
-        if (x->ring_fill == x->n_tb_upd || N * (run_n + 1) > kMaxCopyRows) MIXER_TRY(flush());
-        if (x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
+        return flush();
+    };
+    int st = body();
+    if (!st && slots_taken > 0) {  // the slots of this call are free once its copies are done
+        hipEvent_t e = nullptr;
+        if (!x->slot_pool.empty()) {
+            e = x->slot_pool.back();
+            x->slot_pool.pop_back();
+        } else {
+            st = hip_status(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+        }
+        if (!st) st = hip_status(hipEventRecord(e, x->s_copy));
+        if (!st) x->slot_inflight.push_back({e, slots_taken});
     }
-    MIXER_TRY(flush());
-    if (token) MIXER_TRY(new_token(x, x->s_copy, token));
-    { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
-    return MIXER_OK;
+    if (!st && token) st = new_token(x, x->s_copy, token);
+    std::lock_guard<std::mutex> lk(x->mtx);
+    x->st.heaps_in += d_heaps;
+    x->st.imissed += d_missed;
+    x->st.seq_gaps += d_gaps;
+    x->st.misalign += d_misalign;
+    poll_pending(x);
+    return st;
 }
 
 int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arrival_ns,
@@ -431,51 +534,61 @@ int mixer_submit_device(mixer_ctx* x, const uint8_t* d, int n_tb, uint64_t arriv
     x->last_arrival_ns = arrival_ns;
     int done = 0;
     bool copied = false;  // a copy of the caller's buffer is queued on s_copy
-    // complete a partly filled ring buffer first
-    while (done < n_tb && x->ring_fill > 0) {
-        copied = true;
-        const int take = std::min(n_tb - done, x->n_tb_upd - x->ring_fill);
-        MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur] + (size_t)x->ring_fill * x->tb_bytes,
-                                 d + (size_t)done * x->tb_bytes, (size_t)take * x->tb_bytes,
-                                 hipMemcpyDeviceToDevice, x->s_copy));
-        done += take;
-        x->ring_fill += take;
-        x->st.heaps_in += (uint64_t)take * x->cfg.n_elem;
-        if (x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
-    }
-    // whole V-updates straight from the caller's buffer
-    while (n_tb - done >= x->n_tb_upd) {
-        if (!(x->cfg.flags & MIXER_CFG_NOFLOP)) {
-            MIXER_TRY(run_update(x, d + (size_t)done * x->tb_bytes));
-            MIXER_TRY(run_gemm(x, d + (size_t)done * x->tb_bytes));
+    uint64_t d_heaps = 0;
+    auto body = [&]() -> int {
+        // complete a partly filled ring buffer first
+        while (done < n_tb && x->ring_fill > 0) {
+            copied = true;
+            const int take = std::min(n_tb - done, x->n_tb_upd - x->ring_fill);
+            MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur] + (size_t)x->ring_fill * x->tb_bytes,
+                                     d + (size_t)done * x->tb_bytes, (size_t)take * x->tb_bytes,
+                                     hipMemcpyDeviceToDevice, x->s_copy));
+            done += take;
+            x->ring_fill += take;
+            d_heaps += (uint64_t)take * x->cfg.n_elem;
+            if (x->ring_fill == x->n_tb_upd) MIXER_TRY(dispatch_ring(x));
         }
-        done += x->n_tb_upd;
-        x->st.heaps_in += (uint64_t)x->n_tb_upd * x->cfg.n_elem;
-    }
-    // leftovers into the ring
-    if (done < n_tb) {
-        const int take = n_tb - done;
-        copied = true;
-        MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur], d + (size_t)done * x->tb_bytes,
-                                 (size_t)take * x->tb_bytes, hipMemcpyDeviceToDevice, x->s_copy));
-        x->ring_fill = take;
-        x->st.heaps_in += (uint64_t)take * x->cfg.n_elem;
-    }
-    // the caller's buffer is free once both streams are past it (a cross-stream wait only when
-    // something was copied: it costs the compute stream an idle gap of tens of microseconds)
-    if (copied) {
-        MIXER_HIP(hipEventRecord(x->copy_done, x->s_copy));
-        MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
-    }
-    if (token) MIXER_TRY(new_token(x, x->s_comp, token));
-    { std::lock_guard<std::mutex> lk(x->mtx); poll_pending(x); }
-    return MIXER_OK;
+        // whole V-updates straight from the caller's buffer
+        while (n_tb - done >= x->n_tb_upd) {
+            if (!(x->cfg.flags & MIXER_CFG_NOFLOP)) {
+                MIXER_TRY(run_update(x, d + (size_t)done * x->tb_bytes));
+                MIXER_TRY(run_gemm(x, d + (size_t)done * x->tb_bytes));
+            }
+            done += x->n_tb_upd;
+            d_heaps += (uint64_t)x->n_tb_upd * x->cfg.n_elem;
+        }
+        // leftovers into the ring
+        if (done < n_tb) {
+            const int take = n_tb - done;
+            copied = true;
+            MIXER_TRY(ring_copy_wait(x));
+            MIXER_HIP(hipMemcpyAsync(x->d_ring[x->ring_cur], d + (size_t)done * x->tb_bytes,
+                                     (size_t)take * x->tb_bytes, hipMemcpyDeviceToDevice, x->s_copy));
+            x->ring_fill = take;
+            d_heaps += (uint64_t)take * x->cfg.n_elem;
+        }
+        // the caller's buffer is free once both streams are past it (a cross-stream wait only
+        // when something was copied: it costs the compute stream an idle gap of tens of
+        // microseconds)
+        if (copied) {
+            MIXER_HIP(hipEventRecord(x->copy_done, x->s_copy));
+            MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->copy_done, 0));
+        }
+        if (token) MIXER_TRY(new_token(x, x->s_comp, token));
+        return MIXER_OK;
+    };
+    const int st = body();
+    std::lock_guard<std::mutex> lk(x->mtx);
+    x->st.heaps_in += d_heaps;
+    poll_pending(x);
+    return st;
 }
 
 int mixer_token_done(mixer_ctx* x, uint64_t t) {
     if (!x || t == 0 || t >= x->tok_next) return MIXER_ERR_ARG;
-    if (x->tok_next - t > (uint64_t)mixer_ctx::NTOK) return 1;  // recycled: long complete
-    hipError_t e = hipEventQuery(x->tok_ev[t % mixer_ctx::NTOK]);
+    const int k = (int)(t % mixer_ctx::NTOK);
+    if (x->tok_owner[k] != t) return 1;  // re-recorded by a later token only once t completed
+    hipError_t e = hipEventQuery(x->tok_ev[k]);
     if (e == hipSuccess) return 1;
     if (e == hipErrorNotReady) return 0;
     return hip_status(e);
@@ -483,8 +596,17 @@ int mixer_token_done(mixer_ctx* x, uint64_t t) {
 
 int mixer_token_wait(mixer_ctx* x, uint64_t t) {
     if (!x || t == 0 || t >= x->tok_next) return MIXER_ERR_ARG;
-    if (x->tok_next - t > (uint64_t)mixer_ctx::NTOK) return MIXER_OK;
-    return hip_status(hipEventSynchronize(x->tok_ev[t % mixer_ctx::NTOK]));
+    const int k = (int)(t % mixer_ctx::NTOK);
+    if (x->tok_owner[k] != t) return MIXER_OK;
+    return hip_status(hipEventSynchronize(x->tok_ev[k]));
+}
+
+// Bounded ingest: the reference's fixed pool of NSLOT = 96 slots (src/rx_host_corr.cu:46,
+// 120-121,139-140,154).
+int mixer_set_ingest_slots(mixer_ctx* x, int n_slots) {
+    if (!x || n_slots < 0) return MIXER_ERR_ARG;
+    x->slots_max = n_slots;
+    return MIXER_OK;
 }
 
 int mixer_sync(mixer_ctx* x) {
@@ -543,6 +665,36 @@ int mixer_read_image(mixer_ctx* x, float* img, float* vmax, float* vmin, uint64_
     return MIXER_OK;
 }
 
+// The reference's monitor frame: the image of dV, all that was integrated since its previous
+// frame (src/rx_host_corr.cu:218-222), from the exact int64 frame grid.
+int mixer_read_frame(mixer_ctx* x, float* img, float* vmax, float* vmin, uint64_t* n_in_frame) {
+    if (!x) return MIXER_ERR_ARG;
+    if (!x->d_Gf) return MIXER_ERR_STATE;
+    MIXER_HIP(hipSetDevice(x->device));
+    const size_t C = x->cfg.n_chan;
+    {
+        std::lock_guard<std::mutex> lk(x->mtx);
+        const uint64_t n = x->n_integ.load(), nf = n - x->frame_mark;
+        if (n_in_frame) *n_in_frame = nf;
+        if (nf == 0) return MIXER_OK;
+        x->frame_mark = n;
+        // on the compute stream, behind the integrations it covers: image the frame grid (and
+        // clear it), once the previous frame's copy has read the frame image buffer
+        MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pubf_done, 0));
+        float* d = x->d_imgf;
+        MIXER_HIP(launch_image(x->s_comp, x->d_Gf, x->d_tw, d, d + x->img_floats, d + x->img_floats + C,
+                               x->cfg.n_chan, x->cfg.ng));
+        MIXER_HIP(hipEventRecord(x->frame_ev, x->s_comp));
+        MIXER_HIP(hipStreamWaitEvent(x->s_pub, x->frame_ev, 0));
+        if (img) MIXER_HIP(hipMemcpyAsync(img, d, x->img_floats * 4, hipMemcpyDeviceToHost, x->s_pub));
+        if (vmax) MIXER_HIP(hipMemcpyAsync(vmax, d + x->img_floats, C * 4, hipMemcpyDeviceToHost, x->s_pub));
+        if (vmin) MIXER_HIP(hipMemcpyAsync(vmin, d + x->img_floats + C, C * 4, hipMemcpyDeviceToHost, x->s_pub));
+        MIXER_HIP(hipEventRecord(x->pubf_done, x->s_pub));
+    }
+    MIXER_HIP(hipStreamSynchronize(x->s_pub));
+    return MIXER_OK;
+}
+
 size_t mixer_corr32_bytes(const mixer_array_cfg* c) {
     return 24 + 4 * ((size_t)c->n_chan * c->ng * c->ng + 2 * (size_t)c->n_chan);
 }
@@ -569,8 +721,12 @@ int mixer_publish_image(mixer_ctx* x, void* region, size_t bytes) {
     const size_t C = x->cfg.n_chan;
     std::vector<float> img(x->img_floats + 2 * C);
     uint64_t n = 0;
-    MIXER_TRY(mixer_read_image(x, img.data(), img.data() + x->img_floats,
-                               img.data() + x->img_floats + C, &n));
+    if (x->d_Gf)  // dV since the previous frame, as the reference's monitor publishes
+        MIXER_TRY(mixer_read_frame(x, img.data(), img.data() + x->img_floats,
+                                   img.data() + x->img_floats + C, &n));
+    else
+        MIXER_TRY(mixer_read_image(x, img.data(), img.data() + x->img_floats,
+                                   img.data() + x->img_floats + C, &n));
     if (n == 0) return MIXER_OK;
     uint64_t* ws = (uint64_t*)((uint8_t*)region + 16);
     const uint64_t frame = ++x->frames_published;
@@ -596,6 +752,7 @@ int mixer_get_stats(mixer_ctx* x, mixer_stats* s) {
 int mixer_set_timing(mixer_ctx* x, int enable) {
     if (!x) return MIXER_ERR_ARG;
     MIXER_HIP(hipSetDevice(x->device));
+    std::lock_guard<std::mutex> lk(x->tmtx);
     MIXER_TRY(resolve_timing(x, true));
     for (int i = 0; i < 4; ++i) { x->t_ms[i] = 0; x->t_n[i] = 0; }
     x->timing = (unsigned)enable & 0xFu;
@@ -605,6 +762,7 @@ int mixer_set_timing(mixer_ctx* x, int enable) {
 int mixer_get_timing(mixer_ctx* x, int stage, double* total_ms, uint64_t* launches) {
     if (!x || stage < 0 || stage > 3) return MIXER_ERR_ARG;
     MIXER_HIP(hipSetDevice(x->device));
+    std::lock_guard<std::mutex> lk(x->tmtx);
     MIXER_TRY(resolve_timing(x, true));
     if (total_ms) *total_ms = x->t_ms[stage];
     if (launches) *launches = x->t_n[stage];

@@ -5,6 +5,9 @@
 
 namespace mixer {
 
+// Tiles per integrate work unit (a run along one tile diagonal).
+constexpr int kSegTiles = 2;
+
 // How the correlator covers the lower triangle of one channel.
 //   QUAD     arrays of a multiple of 4 blocks (N = 128, 256, 384, ..., 1024) staged straight
 //            from the heaps (prefetch 1 or 3): one workgroup per (4 adjacent channels, item of
@@ -70,8 +73,9 @@ hipError_t launch_integrate(hipStream_t s, const XcorrPlan& p, const int32_t* ac
                             float* v_add, unsigned long long* G, int N, int C, int grid_x, int ng);
 size_t image_lds_bytes(int ng);
 hipError_t set_image_lds_limit(int ng);
+// image of the int64 grid G (cleared); Gf (optional) accumulates G for a frame image
 hipError_t launch_image(hipStream_t s, unsigned long long* G, const float2* tw, float* img,
-                        float* vmax, float* vmin, int C, int ng);
+                        float* vmax, float* vmin, int C, int ng, unsigned long long* Gf = nullptr);
 hipError_t launch_xcorr_plain(hipStream_t s, const int8_t* tiles, int32_t* vre, int32_t* vim, int N,
                               int n_tb, int c);
 

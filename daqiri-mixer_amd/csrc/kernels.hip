@@ -39,16 +39,17 @@ __global__ __launch_bounds__(256) void k_integrate(const int32_t* __restrict__ a
 }
 
 __global__ __launch_bounds__(256) void k_image(unsigned long long* __restrict__ G,
+                                               unsigned long long* __restrict__ Gf,
                                                const float2* __restrict__ tw,
                                                float* __restrict__ img, float* __restrict__ vmax,
                                                float* __restrict__ vmin, int ng) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    image_body(G, tw, img, vmax, vmin, ng, blockIdx.x, smem);
+    image_body(G, Gf, tw, img, vmax, vmin, ng, blockIdx.x, smem);
 }
 
 // ------------------------------------------------------------------------------------------
 // Plain form: one wave per 32x32 output tile of the FULL square of one channel, four MFMAs
-// per 32-sample step exactly as cgemm.cuh:115-118, written to V[N][N] (overwrite).
+// per 32-sample step exactly as cgemm.cuh:59-62, written to V[N][N] (overwrite).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_xcorr_plain(const int8_t* __restrict__ tiles,
                                                      int32_t* __restrict__ vre,
@@ -135,9 +136,9 @@ hipError_t launch_integrate(hipStream_t s, const XcorrPlan& p, const int32_t* ac
 size_t image_lds_bytes(int ng) { return (size_t)ng * ng * 16; }
 
 hipError_t launch_image(hipStream_t s, unsigned long long* G, const float2* tw, float* img,
-                        float* vmax, float* vmin, int C, int ng) {
-    hipLaunchKernelGGL(k_image, dim3(C), dim3(256), image_lds_bytes(ng), s, G, tw, img, vmax, vmin,
-                       ng);
+                        float* vmax, float* vmin, int C, int ng, unsigned long long* Gf) {
+    hipLaunchKernelGGL(k_image, dim3(C), dim3(256), image_lds_bytes(ng), s, G, Gf, tw, img, vmax,
+                       vmin, ng);
     return hipGetLastError();
 }
 

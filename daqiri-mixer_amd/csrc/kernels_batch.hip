@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_batch_image(const BatchJobDev* __restri
     const int j = job_of(jobs, n_jobs, b, [](const BatchJobDev& J) { return J.img_wg0; });
     const BatchJobDev& J = jobs[j];
     const size_t nimg = (size_t)J.n_chan * J.ng * J.ng;
-    image_body(J.G, J.tw, J.img, J.img + nimg, J.img + nimg + J.n_chan, J.ng, b - J.img_wg0, smem);
+    image_body(J.G, nullptr, J.tw, J.img, J.img + nimg, J.img + nimg + J.n_chan, J.ng, b - J.img_wg0, smem);
 }
 
 hipError_t launch_batch_reset(hipStream_t s, int* counters) {

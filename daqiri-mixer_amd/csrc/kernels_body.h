@@ -191,6 +191,15 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
 // published as fftshift + real part + per-channel max/min. G[c] is zeroed for the next
 // integration.
 // ------------------------------------------------------------------------------------------
+// Frame grid (optional): the exact int64 grid summed over the integrations since the last frame
+// (the reference's monitor images dV, the change since its previous frame,
+// src/rx_host_corr.cu:218-222). One workgroup owns a channel's cells, so a plain add suffices.
+__device__ __forceinline__ void frame_add(unsigned long long* Gfc, int q, unsigned long long re,
+                                          unsigned long long im) {
+    gst(&Gfc[2 * q], gld(&Gfc[2 * q]) + re);
+    gst(&Gfc[2 * q + 1], gld(&Gfc[2 * q + 1]) + im);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -234,6 +243,7 @@ __device__ __forceinline__ void fft32_line(float2 v[4], int r, const float2* __r
 }
 
 __device__ __forceinline__ void image32_body(unsigned long long* __restrict__ G,
+                                             unsigned long long* __restrict__ Gf,
                                              const float2* __restrict__ tw, float* __restrict__ img,
                                              float* __restrict__ vmax, float* __restrict__ vmin,
                                              int c, unsigned char* smem) {
@@ -248,9 +258,11 @@ __device__ __forceinline__ void image32_body(unsigned long long* __restrict__ G,
 #pragma unroll
     for (int m = 0; m < 4; ++m) {  // row y = line, column x = r + 8 m
         const int q = line * ng + r + 8 * m;
-        v[m] = make_float2((float)(long long)gld(&Gc[2 * q]), (float)(long long)gld(&Gc[2 * q + 1]));
+        const unsigned long long gr = gld(&Gc[2 * q]), gi = gld(&Gc[2 * q + 1]);
+        v[m] = make_float2((float)(long long)gr, (float)(long long)gi);
         gst(&Gc[2 * q], 0ull);
         gst(&Gc[2 * q + 1], 0ull);
+        if (Gf) frame_add(Gf + (size_t)c * 2 * ng * ng, q, gr, gi);
     }
     fft32_line(v, r, tw);
 #pragma unroll
@@ -285,11 +297,12 @@ __device__ __forceinline__ void image32_body(unsigned long long* __restrict__ G,
 }
 
 __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
+                                           unsigned long long* __restrict__ Gf,
                                            const float2* __restrict__ tw, float* __restrict__ img,
                                            float* __restrict__ vmax, float* __restrict__ vmin,
                                            int ng, int c, unsigned char* smem) {
     if (ng == 32) {
-        image32_body(G, tw, img, vmax, vmin, c, smem);
+        image32_body(G, Gf, tw, img, vmax, vmin, c, smem);
         return;
     }
     const int nn = ng * ng;
@@ -299,9 +312,11 @@ __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
     const int tid = threadIdx.x;
     unsigned long long* Gc = G + (size_t)c * 2 * nn;
     for (int q = tid; q < nn; q += 256) {
-        F0[q] = make_float2((float)(long long)gld(&Gc[2 * q]), (float)(long long)gld(&Gc[2 * q + 1]));
+        const unsigned long long gr = gld(&Gc[2 * q]), gi = gld(&Gc[2 * q + 1]);
+        F0[q] = make_float2((float)(long long)gr, (float)(long long)gi);
         gst(&Gc[2 * q], 0ull);
         gst(&Gc[2 * q + 1], 0ull);
+        if (Gf) frame_add(Gf + (size_t)c * 2 * nn, q, gr, gi);
     }
     __syncthreads();
     float2* src = F0;

@@ -96,10 +96,10 @@ std::vector<std::pair<int, int>> quad_items_for(int nb) {
     return v;
 }
 
-// Integrate work units: runs of at most kSegTiles tiles (J + d, J) along tile diagonal d. A
-// workgroup loads its tiles one after another, so short runs keep the longest chain short
-// (256 x 128: 43 -> 35 us going from 8 to 2 tiles; 1024 x 256: 660 -> 632 us).
-constexpr int kSegTiles = 2;
+// Integrate work units: runs of at most kSegTiles tiles (J + d, J) along tile diagonal d (d
+// ascending, then j0 ascending). A workgroup loads its tiles one after another, so short runs
+// keep the longest chain short (256 x 128: 43 -> 35 us going from 8 to 2 tiles; 1024 x 256:
+// 660 -> 632 us).
 std::vector<int4> segs_for(int nb) {
     std::vector<int4> v;
     for (int d = 0; d < nb; ++d)

@@ -72,8 +72,11 @@ struct SrvArray {
     hipEvent_t ev_launch = nullptr;    // after the last correlator launch
     uint64_t tok_launched = 0;         // tokens up to this one have their events recorded
     uint64_t n_integ = 0;  // integrations enqueued
+    // token t uses slot t % NTOK; a slot's event is re-recorded only after the previous token's
+    // event has completed, so a token whose slot holds a later token is complete
     static constexpr int NTOK = 64;
     hipEvent_t tok[NTOK] = {};
+    uint64_t tok_owner[NTOK] = {};
     uint64_t tok_next = 1;
     std::vector<void*> dev;
 };
@@ -284,9 +287,12 @@ int launch_run(mixer_server* S, SrvArray* a) {
         a->run_ptr = nullptr;
     }
     for (uint64_t t = a->tok_launched + 1; t < a->tok_next; ++t) {
-        hipEvent_t& e = a->tok[t % SrvArray::NTOK];
+        const int k = (int)(t % SrvArray::NTOK);
+        hipEvent_t& e = a->tok[k];
         if (!e) MIXER_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+        else MIXER_HIP(hipEventSynchronize(e));  // the slot's previous token completes first
         MIXER_HIP(hipEventRecord(e, a->s));
+        a->tok_owner[k] = t;
     }
     a->tok_launched = a->tok_next - 1;
     return MIXER_OK;
@@ -470,12 +476,11 @@ int mixer_server_submit(mixer_server* S, int id, const uint8_t* d, int n_tb, uin
 int mixer_server_token_wait(mixer_server* S, int id, uint64_t t) {
     SrvArray* a = S ? get_array(S, id) : nullptr;
     if (!a || t == 0 || t >= a->tok_next) return MIXER_ERR_ARG;
-    if (a->tok_next - t > (uint64_t)SrvArray::NTOK) return MIXER_OK;  // recycled: long complete
-    if (t > a->tok_launched) {  // still held back: launch it now
-        MIXER_HIP(hipSetDevice(S->device));
-        MIXER_TRY(launch_run(S, a));
-    }
-    return hip_status(hipEventSynchronize(a->tok[t % SrvArray::NTOK]));
+    MIXER_HIP(hipSetDevice(S->device));
+    if (t > a->tok_launched) MIXER_TRY(launch_run(S, a));  // still held back: launch it now
+    const int k = (int)(t % SrvArray::NTOK);
+    if (a->tok_owner[k] != t) return MIXER_OK;  // re-recorded by a later token once t completed
+    return hip_status(hipEventSynchronize(a->tok[k]));
 }
 
 int mixer_server_set_coalescing(mixer_server* S, int enable) {

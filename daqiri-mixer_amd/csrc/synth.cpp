@@ -186,4 +186,15 @@ int mixer_synth_packets(const mixer_array_cfg* cfg, const mixer_sky* sky, uint64
     return MIXER_OK;
 }
 
+// Orbiting planet of the reference transmitter's sky-update thread (src/tx_host.cu:98-100,
+// 116-120): (l, m) = rho (cos w t, sin w t), w = 2 pi / torbit, t seconds since motion started.
+int mixer_orbit_sky(const mixer_sky* in, double rho, double torbit_s, double t_s, mixer_sky* out) {
+    if (!in || !out || !(torbit_s > 0)) return MIXER_ERR_ARG;
+    const double w = 2.0 * M_PI / torbit_s;
+    *out = *in;
+    out->planet_l = rho * std::cos(w * t_s);
+    out->planet_m = rho * std::sin(w * t_s);
+    return MIXER_OK;
+}
+
 }  // extern "C"

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmixer.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BUDGET, ERR_NODEV, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
-CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP, CFG_AUTOTUNE = 1, 2, 4, 8
+CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP, CFG_AUTOTUNE, CFG_FRAME_DV = 1, 2, 4, 8, 16
 H2D, D2H, D2D = 0, 1, 2
 
 
@@ -132,11 +132,13 @@ SIGNATURES = [
     ("mixer_get_cfg", I32, [P, P]),
     ("mixer_submit_heaps", I32, [P, P, SZ, I32, U64, P]),
     ("mixer_submit_device", I32, [P, P, I32, U64, P]),
+    ("mixer_set_ingest_slots", I32, [P, I32]),
     ("mixer_token_done", I32, [P, U64]),
     ("mixer_token_wait", I32, [P, U64]),
     ("mixer_sync", I32, [P]),
     ("mixer_dump", I32, [P, P, I32, P]),
     ("mixer_read_image", I32, [P, P, P, P, P]),
+    ("mixer_read_frame", I32, [P, P, P, P, P]),
     ("mixer_corr32_bytes", SZ, [P]),
     ("mixer_corr32_init", I32, [P, P]),
     ("mixer_publish_image", I32, [P, P, SZ]),
@@ -190,6 +192,7 @@ SIGNATURES = [
     ("mixer_default_wire", None, [P]),
     ("mixer_build_header", I32, [P, P, C.c_uint32, P]),
     ("mixer_synth_set_wire", I32, [P]),
+    ("mixer_orbit_sky", I32, [P, C.c_double, C.c_double, C.c_double, P]),
     ("mixer_synth_packets", I32, [P, P, U64, U64, I32, P, SZ, I32]),
 ]
 
@@ -241,6 +244,13 @@ def default_sky(**kw) -> Sky:
     return s
 
 
+def orbit_sky(sky: Sky, rho: float, torbit_s: float, t_s: float) -> Sky:
+    """The sky with its planet moved along the transmitter's orbit to time t_s."""
+    out = Sky()
+    check(lib().mixer_orbit_sky(C.byref(sky), rho, torbit_s, t_s, C.byref(out)), "mixer_orbit_sky")
+    return out
+
+
 def default_wire() -> Wire:
     w = Wire()
     lib().mixer_default_wire(C.byref(w))
@@ -268,6 +278,14 @@ def synth_packets(cfg: ArrayCfg, sky: Sky, seed: int, seq0: int, npkts: int, pit
     out = np.zeros((npkts, pitch), np.uint8)
     check(lib().mixer_synth_packets(C.byref(cfg), C.byref(sky), seed, seq0, npkts, ptr(out), pitch, nthreads),
           "mixer_synth_packets")
+    return out
+
+
+def download_ptr(addr: int, dtype, shape) -> np.ndarray:
+    """Copy device memory at a raw device address to a new host array."""
+    out = np.empty(shape, dtype)
+    check(lib().mixer_memcpy(ptr(out), C.c_void_p(addr), out.nbytes, D2H, None), "mixer_memcpy")
+    check(lib().mixer_stream_sync(None), "mixer_stream_sync")
     return out
 
 
@@ -437,6 +455,16 @@ class Engine:
     def token_wait(self, tok: int):
         check(lib().mixer_token_wait(self.h, tok), "mixer_token_wait")
 
+    def token_done(self, tok: int) -> bool:
+        r = lib().mixer_token_done(self.h, tok)
+        if r < 0:
+            check(r, "mixer_token_done")
+        return r == 1
+
+    def set_ingest_slots(self, n_slots: int):
+        """Bounded ingest: a time block without a free slot is dropped and counted (imissed)."""
+        check(lib().mixer_set_ingest_slots(self.h, n_slots), "mixer_set_ingest_slots")
+
     def sync(self):
         check(lib().mixer_sync(self.h), "mixer_sync")
 
@@ -455,6 +483,16 @@ class Engine:
         idx = C.c_uint64(0)
         check(lib().mixer_read_image(self.h, ptr(img), ptr(vmax), ptr(vmin), C.byref(idx)), "mixer_read_image")
         return img, vmax, vmin, idx.value
+
+    def read_frame(self):
+        """(img, vmax, vmin, n): the image of everything integrated since the previous frame."""
+        c = self.cfg
+        img = np.empty((c.n_chan, c.ng, c.ng), np.float32)
+        vmax = np.empty(c.n_chan, np.float32)
+        vmin = np.empty(c.n_chan, np.float32)
+        n = C.c_uint64(0)
+        check(lib().mixer_read_frame(self.h, ptr(img), ptr(vmax), ptr(vmin), C.byref(n)), "mixer_read_frame")
+        return img, vmax, vmin, n.value
 
     def publish(self, region: np.ndarray):
         check(lib().mixer_publish_image(self.h, ptr(region), region.nbytes), "mixer_publish_image")
@@ -535,6 +573,12 @@ class Batch:
         out = np.empty((c.n_chan, 2, c.n_elem, c.n_elem), np.float32)
         check(lib().mixer_batch_dump(self.h, job, ptr(out)), "mixer_batch_dump")
         return out
+
+    def acc(self, job: int):
+        """(device address, time slots) of job's int32 accumulator of the last run."""
+        p, n = C.c_void_p(), C.c_int(0)
+        check(lib().mixer_batch_acc(self.h, job, C.byref(p), C.byref(n)), "mixer_batch_acc")
+        return p.value, n.value
 
     def read_image(self, job: int):
         c = self.cfgs[job]

@@ -4,7 +4,7 @@
  * four seams, each replaced here by extern "C" entry points with plain pointers and sizes:
  *   input seam   Slot{mr_ptr} of 256 packets at pitch MR_STRIDE   src/rx_host_corr.cu:120-121,136-141,159
  *   kernel seam  k_packAB(bb, Are, Aim, Bre, Bim)                  src/rx_host_corr.cu:61-72
- *                cgemm_blk_t<ACC>(M,N,K,Are,Aim,Bre,Bim,Cre,Cim)  src/cgemm.cuh:84-125
+ *                cgemm_blk_t<ACC>(M,N,K,Are,Aim,Bre,Bim,Cre,Cim)  src/cgemm.cuh:28-69
  *                k_flush_cube(Vcube, Vire, Viim)                   src/rx_host_corr.cu:90-95
  *                k_diff / k_grid_cube / cufftExecC2C               src/rx_host_corr.cu:74-85,205-206,222
  *   output seam  --dump record [C][2][N][N] f32                    src/rx_host_corr.cu:187,210-227
@@ -77,6 +77,10 @@ typedef struct mixer_array_cfg {
                                   src/rx_host_corr.cu:161) */
 #define MIXER_CFG_AUTOTUNE 8   /* mixer_create runs mixer_autotune when the size class has no
                                   tuning yet */
+#define MIXER_CFG_FRAME_DV 16  /* keep an exact frame grid: mixer_read_frame returns the image of
+                                  everything integrated since the previous frame (the reference's
+                                  dV monitor, src/rx_host_corr.cu:218-222), and
+                                  mixer_publish_image publishes it */
 
 void mixer_default_cfg(mixer_array_cfg* cfg);
 int mixer_validate_cfg(const mixer_array_cfg* cfg);
@@ -102,7 +106,7 @@ size_t mixer_tiles_bytes(const mixer_array_cfg* cfg, int n_tb);
 int mixer_corner_turn(void* stream, const mixer_array_cfg* cfg, const uint8_t* d_heaps, int n_tb,
                       int8_t* d_tiles, uint32_t* d_wrap);
 
-/* Correlate: replaces cgemm_blk_t<ACC> (src/cgemm.cuh:84-125) on the lower triangle.
+/* Correlate: replaces cgemm_blk_t<ACC> (src/cgemm.cuh:28-69) on the lower triangle.
  *   d_acc    int32 accumulator, mixer_acc_bytes(cfg) bytes: [n_slots][n_chan][n_tiles][2][32][32]
  *            with tile t = I*(I+1)/2 + J for 32x32 element blocks I >= J, plane 0 = Re, 1 = Im,
  *            row = element of block I, column = element of block J, and n_slots partial sums
@@ -121,7 +125,7 @@ int mixer_integrate(void* stream, const mixer_array_cfg* cfg, const int32_t* d_a
                     int accumulate);
 
 /* Fused corner-turn + correlate: k_packAB + cgemm_blk_t<ACC> (src/rx_host_corr.cu:61-72,
- * src/cgemm.cuh:84-125) in one launch. The correlator stages each 32-sample element row
+ * src/cgemm.cuh:28-69) in one launch. The correlator stages each 32-sample element row
  * straight from the header-free heaps d_heaps [n_tb][n_elem][re C*32 | im C*32] (every row is
  * one 32-byte run of a heap plane), so no tiles are written or read. Same accumulator as
  * mixer_xcorr. d_wrap (optional) counts workgroups that staged an imaginary -128. */
@@ -165,7 +169,7 @@ int mixer_get_tuning(const mixer_array_cfg* cfg, mixer_tuning* t); /* the plan i
 int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuning* best);
 
 /* Plain per-channel-launch form (baseline): one corner-turn launch and one full-square GEMM
- * launch per channel, exactly the reference's 4-real-product math (cgemm.cuh:114-119) on the
+ * launch per channel, exactly the reference's 4-real-product math (cgemm.cuh:58-63) on the
  * whole square, no lower-triangle saving, no persistence. Output d_vre/d_vim [n_chan][N][N]
  * int32 (overwrite). d_tiles is scratch of mixer_tiles_bytes(cfg, n_tb). */
 int mixer_xcorr_plain(void* stream, const mixer_array_cfg* cfg, const uint8_t* d_heaps, int n_tb,
@@ -201,9 +205,20 @@ int mixer_get_cfg(mixer_ctx* ctx, mixer_array_cfg* cfg);
  * packets, within one ring buffer) on the context's copy stream. *token receives a completion
  * token: the caller's buffer is free for reuse once mixer_token_done/wait reports it (the
  * reference returns its packets to the NIC before the copy has run; this API does not).
- * arrival_ns (steady clock) of the last packet feeds the dump-latency statistics. */
+ * At most 256 tokens are outstanding per context: a call that asks for a token while the
+ * token 256 before it is still pending first waits for that one, so a token reported done
+ * is always complete. arrival_ns (steady clock) of the last packet feeds the dump-latency
+ * statistics. */
 int mixer_submit_heaps(mixer_ctx* ctx, const uint8_t* pkts, size_t pitch, int npkts,
                        uint64_t arrival_ns, uint64_t* token);
+/* Bounded ingest, the reference's fixed slot pool (NSLOT = 96, src/rx_host_corr.cu:46,120-121):
+ * with n_slots > 0 every time block that mixer_submit_heaps accepts holds one slot until its
+ * header-strip copy has completed, and a time block that finds no free slot is dropped and
+ * counted in mixer_stats.imissed (src/rx_host_corr.cu:139-140). Slots are returned at the start
+ * of each mixer_submit_heaps call whose copies have completed by then (the reference's
+ * gpu_thread polls its in-flight slots the same way, src/rx_host_corr.cu:154). 0 (the default)
+ * is unbounded: nothing is dropped and the copy stream queues without limit. */
+int mixer_set_ingest_slots(mixer_ctx* ctx, int n_slots);
 /* Device-resident input: header-stripped payloads [n_tb][n_elem][payload] already in HBM
  * (the replayed-ring benchmark path). The buffer must stay valid until the token completes.
  * For both submit calls token may be NULL: then no completion event is recorded (an event
@@ -224,10 +239,17 @@ int mixer_sync(mixer_ctx* ctx);                       /* wait for all submitted 
 int mixer_dump(mixer_ctx* ctx, float* host_out, int cumulative, uint64_t* integ_index);
 /* The latest image cube: img [n_chan][ng][ng], vmax/vmin [n_chan] (any may be NULL). */
 int mixer_read_image(mixer_ctx* ctx, float* img, float* vmax, float* vmin, uint64_t* integ_index);
+/* Monitor frame (needs MIXER_CFG_FRAME_DV): the image of dV, the sum of V over all integrations
+ * completed since the previous mixer_read_frame (the reference images dV = cube - previous
+ * frame's cube, src/rx_host_corr.cu:218-222), gridded in exact int64 and transformed once.
+ * *n_in_frame receives the number of integrations the frame covers (0: nothing copied). */
+int mixer_read_frame(mixer_ctx* ctx, float* img, float* vmax, float* vmin, uint64_t* n_in_frame);
 
 /* Corr32 shared-memory image cube (src/rx_host_corr.cu:97-105): {u32 magic=0x43523343, w, h,
  * nch; u64 write_seq; f32 img[nch*h*w]; f32 vmax[nch]; f32 vmin[nch]} = 525336 bytes at the
- * default geometry. write_seq has bit 63 set while a frame is being written and holds
+ * default geometry. mixer_publish_image publishes the latest integration's image, or with
+ * MIXER_CFG_FRAME_DV the frame since the previous publish (nothing when no integration
+ * completed since). write_seq has bit 63 set while a frame is being written and holds
  * frame+1 (release order) once it is complete, so readers that check the bit never see a torn
  * frame and the reference's peek32 viewer reads the region unchanged. */
 size_t mixer_corr32_bytes(const mixer_array_cfg* cfg);
@@ -295,8 +317,10 @@ int mixer_batch_dump(mixer_batch* b, int job, float* host_v);
 int mixer_batch_read_image(mixer_batch* b, int job, float* img, float* vmax, float* vmin);
 int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots);
 /* Autotuner for a batch: times runs of b on its current inputs (the last d_heaps given to
- * mixer_batch_run, which must have been called once) for each pack x tb_target candidate and
- * leaves b on the fastest, returned in *best. */
+ * mixer_batch_run, which must have been called once) for each pack x tb_target candidate, each
+ * as the median of reps individually timed runs after a warm-up run, and leaves b on the
+ * fastest, returned in *best. It moves off the tuning b had only for a candidate at least 2 %
+ * faster, so noise cannot leave b slower than it started. */
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best);
 /* Fills n bytes of device memory with seeded uniform int8 values in [-127, 127]. */
 int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed);
@@ -441,6 +465,11 @@ void mixer_default_wire(mixer_wire* w); /* a0:88:c2:0d:5e:28, 10.0.0.1 -> 10.0.0
 int mixer_build_header(const mixer_array_cfg* cfg, const mixer_wire* w, uint32_t seq, uint8_t* hdr);
 /* Header template used by mixer_synth_packets from now on (process-wide). */
 int mixer_synth_set_wire(const mixer_wire* w);
+/* The orbiting planet of the reference transmitter (src/tx_host.cu:98-100,116-120): copies *in to
+ * *out with the planet at (l, m) = rho (cos wt, sin wt), w = 2 pi / torbit_s, t_s seconds after
+ * motion started (t_s = 0: l = rho, m = 0, the transmitter's first frame, src/tx_host.cu:106).
+ * mixer_corr --rho/--torbit/--refresh-hz re-generates its replay slots with it. */
+int mixer_orbit_sky(const mixer_sky* in, double rho, double torbit_s, double t_s, mixer_sky* out);
 
 #ifdef __cplusplus
 }

@@ -123,3 +123,43 @@ def test_wire_header_is_the_transmitters():
     for k in range(3):
         exp = M.build_header(c, 513 + k)
         assert np.array_equal(pk[k, :64], exp)
+
+
+def test_orbit_sky_quarter_periods_and_payload():
+    """The transmitter's orbiting planet (src/tx_host.cu:116-120): at t = 0, T/4, T/2, 3T/4 the
+    planet sits at (rho, 0), (0, rho), (-rho, 0), (0, -rho), counter-clockwise, always at radius
+    rho; the rest of the sky is untouched. A slot regenerated at an orbit time (sigma = 0)
+    carries the oracle's k_sky payload for that planet, and its image puts the planet at pixel
+    (16 + 16 l, 16 + 16 m)."""
+    rho, T = 0.5, 3.0
+    base = M.default_sky(sigma=0.0, star_amp=2.0)
+    want = {0.0: (rho, 0.0), T / 4: (0.0, rho), T / 2: (-rho, 0.0), 3 * T / 4: (0.0, -rho), T: (rho, 0.0)}
+    for t, (l, m) in want.items():
+        s = M.orbit_sky(base, rho, T, t)
+        assert abs(s.planet_l - l) < 1e-12 and abs(s.planet_m - m) < 1e-12, t
+        assert s.star_amp == base.star_amp and s.sigma == base.sigma and s.planet_scale == base.planet_scale
+    prev = None
+    for t in np.linspace(0.01, T / 4 - 0.01, 7):  # first quadrant, angle increasing
+        s = M.orbit_sky(base, rho, T, float(t))
+        assert abs(np.hypot(s.planet_l, s.planet_m) - rho) < 1e-12
+        assert s.planet_l > 0 and s.planet_m > 0
+        ang = np.arctan2(s.planet_m, s.planet_l)
+        assert prev is None or ang > prev
+        prev = ang
+    with pytest.raises(M.MixerError):
+        M.orbit_sky(base, rho, 0.0, 1.0)
+    N, gx, Cn = 256, 16, 128
+    c = M.default_cfg(n_elem=N, grid_x=gx, n_chan=Cn)
+    for t, (px, py) in ((T / 4, (16, 24)), (T / 2, (8, 16))):
+        s = M.orbit_sky(base, rho, T, t)
+        pk = M.synth_packets(c, s, seed=3, seq0=0, npkts=N)
+        for a in (0, 17, 255):
+            assert np.array_equal(pk[a, 64:], O.sky_payload(a, gx, Cn, 32, s.planet_l, s.planet_m)), (t, a)
+        bb = np.ascontiguousarray(pk[:, 64:]).reshape(1, N, 64 * Cn)
+        vre, vim = O.v_full_from_heaps(bb, 1, N, Cn, 32)
+        img = O.image_from_v(vre, vim, gx, 32)[0]
+        ch = Cn - 1  # the planet grows with channel (src/tx_host.cu:48): brightest in the last
+        yy, xx = np.mgrid[0:32, 0:32]
+        off = (xx - 16) ** 2 + (yy - 16) ** 2 > 4
+        oy, ox = np.unravel_index(int(np.argmax(np.where(off, img[ch], -np.inf))), (32, 32))
+        assert (ox, oy) == (px, py), (t, ox, oy)

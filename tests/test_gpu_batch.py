@@ -101,9 +101,12 @@ def test_batch_time_split_and_autotune():
         b.sync()
         for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):
             check_job(b, j, cfg, bb, jb[2])
-        best = b.autotune(reps=2)
+        best = b.autotune(reps=3)
         log(f"  autotune -> {best.as_dict()}")
-        assert best.pack in (0, 1) and best.tb_target in (64, 128, 256, 512)
+        # a candidate, or the starting tuning when no candidate beat it by 2 %
+        assert best.pack in (0, 1) and best.tb_target in (8, 64, 128, 256, 512)
+        if best.tb_target == 8:
+            assert best.pack == t.pack
         b.run(bufs)
         b.sync()
         for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):

@@ -73,3 +73,46 @@ def test_cli_publishes_images_and_dumps_hermitian_cubes(tmp_path):
         assert (np.diagonal(v[:, 0], axis1=1, axis2=2) > 0).all()
     stats = [l for l in err.splitlines() if "imissed=" in l]
     assert stats and all("imissed=0 misalign=0" in l for l in stats)
+
+
+def test_cli_orbiting_planet_moves_along_the_orbit():
+    """--rho/--torbit/--refresh-hz (src/tx_host.cu:98-100,116-122,144-147): the synthetic
+    transmitter regenerates its replay slots with the planet at rho (cos wt, sin wt). The
+    published frames (dV since the previous frame) put the planet, the brightest pixel more than
+    2 pixels off centre in the highest channel, at radius 16 rho = 8 pixels around the centre, and
+    its angle advances counter-clockwise over more than half a turn."""
+    N, C, rho, torbit = 256, 64, 0.5, 4.0
+    shm = f"/mixer_cli_orbit_{os.getpid()}"
+    cmd = [BIN, "--seconds", "6", "--elements", str(N), "--grid-x", "16", "--channels", str(C),
+           "--k-update", "1024", "--flush", "2", "--ng", "32", "--rate", "0.5", "--fps", "5",
+           "--rho", str(rho), "--torbit", str(torbit), "--refresh-hz", "100", "--shm", shm]
+    print(" ".join(cmd), flush=True)
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    pos = {}
+    try:
+        deadline = time.time() + 60
+        while p.poll() is None and time.time() < deadline:
+            fr = read_shm(shm)
+            if fr is not None and fr[0] not in pos:
+                ws, img = fr[0], fr[1][C - 1]
+                yy, xx = np.mgrid[0:32, 0:32]
+                off = (xx - 16) ** 2 + (yy - 16) ** 2 > 4
+                oy, ox = np.unravel_index(int(np.argmax(np.where(off, img, -np.inf))), img.shape)
+                pos[ws] = (int(ox), int(oy))
+            time.sleep(0.05)
+        out, err = p.communicate(timeout=60)
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.communicate()
+    sys.stdout.write(err[-1200:])
+    assert p.returncode == 0, err[-2000:]
+    seq = [pos[k] for k in sorted(pos)]
+    print("planet pixels by frame:", seq, flush=True)
+    assert len(seq) >= 8, seq
+    r = np.array([np.hypot(x - 16, y - 16) for x, y in seq])
+    assert np.mean(np.abs(r - 16 * rho) <= 1.5) >= 0.8, r
+    ang = np.unwrap([np.arctan2(y - 16, x - 16) for x, y in seq])
+    steps = np.diff(ang)
+    assert np.mean(steps > 0) >= 0.7, steps
+    assert ang[-1] - ang[0] > np.pi, ang
