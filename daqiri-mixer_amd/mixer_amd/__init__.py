@@ -123,6 +123,7 @@ SIGNATURES = [
     ("mixer_integrate", I32, [P, P, P, P, I32]),
     ("mixer_xcorr_heaps", I32, [P, P, P, I32, P, I32, P]),
     ("mixer_image", I32, [P, P, P, P, P, P]),
+    ("mixer_image_noalloc", I32, [P, P, P, P, P, P]),
     ("mixer_set_tuning", I32, [I32, P]),
     ("mixer_get_tuning", I32, [P, P]),
     ("mixer_autotune", I32, [P, I32, P]),
@@ -360,6 +361,13 @@ def integrate(cfg, d_acc: DeviceBuffer, d_v: DeviceBuffer, accumulate=False, str
 
 def image(cfg, d_acc: DeviceBuffer, d_img: DeviceBuffer, d_vmax: DeviceBuffer, d_vmin: DeviceBuffer, stream=None):
     check(lib().mixer_image(stream, C.byref(cfg), d_acc.ptr, d_img.ptr, d_vmax.ptr, d_vmin.ptr), "mixer_image")
+
+
+def image_noalloc(cfg, d_acc: DeviceBuffer, d_img: DeviceBuffer, d_vmax: DeviceBuffer, d_vmin: DeviceBuffer,
+                  stream=None):
+    """mixer_image without device allocation (static workspace of the library)."""
+    check(lib().mixer_image_noalloc(stream, C.byref(cfg), d_acc.ptr, d_img.ptr, d_vmax.ptr, d_vmin.ptr),
+          "mixer_image_noalloc")
 
 
 def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):

@@ -140,6 +140,17 @@ int mixer_xcorr_heaps(void* stream, const mixer_array_cfg* cfg, const uint8_t* d
 int mixer_image(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_img,
                 float* d_vmax, float* d_vmin);
 
+/* Image without any device allocation: no counterpart in the reference. For callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, outputs and results as mixer_image, but the int64 grid workspace, the integrate
+ * segment list and the FFT twiddles live in one static device array of the library (16 MiB of
+ * grid plus 4.6 KiB of tables, held on every GPU the library runs on until the process ends):
+ * n_chan * ng * ng * 16 bytes must not exceed 16 MiB (256 channels at ng = 64, 1024 at ng = 32),
+ * or MIXER_ERR_ARG. Every call on a GPU uses that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race. Each call clears the workspace it uses first. */
+int mixer_image_noalloc(void* stream, const mixer_array_cfg* cfg, const int32_t* d_acc, float* d_img,
+                        float* d_vmax, float* d_vmin);
+
 /* Launch tuning (the autotuner's search space), per array size class n_elem:
  *   slots     time split of one correlator call into int32 partial sums (1..8)
  *   prefetch  correlator time-loop pipeline: 1 = register staging one 32-sample block ahead,
