@@ -147,7 +147,7 @@ def bench_batch(reps=5):
         info = b.info()
         start = M.batch_default_tuning()  # the batch was created with the default tuning
         t_def = timed(b)
-        best = b.autotune(reps=5)
+        best = b.autotune(reps=3)
         t_tuned = timed(b)
         info = b.info()
     tot = sum(payload)
@@ -281,6 +281,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.5)
+    ap.add_argument("--staging", type=int, default=0, choices=(0, 1, 2),
+                    help="1: the correlator stages straight from the heaps (automatic choice); "
+                         "2: a separate corner-turn pass into tiles feeds it (to measure the corner-turn)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} not supported)",
@@ -290,6 +293,8 @@ def main():
     M.check(M.lib().mixer_set_device(0))
 
     cfg = make_cfg(args.config)
+    if args.staging:
+        M.set_tuning(cfg.n_elem, staging=args.staging)
     alg = algorithmic(cfg)
     N, C, K = cfg.n_elem, cfg.n_chan, cfg.k_integ
     n_tb = K // 32

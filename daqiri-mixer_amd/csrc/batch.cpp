@@ -353,9 +353,10 @@ int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots
     return MIXER_OK;
 }
 
-// Times every candidate as the median of reps individually timed runs (after one warm-up run)
-// and moves off the starting tuning only for a candidate at least 2 % faster than the best so far,
-// so run-to-run noise cannot make the tuned batch slower than the one it started from.
+// Times every candidate as the median over 3 blocks of reps back-to-back runs (after one warm-up
+// run), as a caller that replays the batch sees it (consecutive runs overlap at their ends, which
+// isolated runs do not show), and moves off the starting tuning only for a candidate at least 2 %
+// faster than the best so far, so run-to-run noise cannot leave the batch slower than it started.
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best) {
     if (!b || !best || reps <= 0) return MIXER_ERR_ARG;
     if (!b->have_inputs) return MIXER_ERR_STATE;
@@ -368,18 +369,17 @@ int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best) {
         MIXER_TRY(plan(b));
         auto run = [&]() { return b->exec ? hip_status(hipGraphLaunch(b->exec, b->s)) : enqueue(b); };
         MIXER_TRY(run());  // warm-up
-        std::vector<float> t;
-        for (int r = 0; r < reps; ++r) {
+        float t[3];
+        for (int k = 0; k < 3; ++k) {
             MIXER_HIP(hipEventRecord(e0, b->s));
-            MIXER_TRY(run());
+            for (int r = 0; r < reps; ++r) MIXER_TRY(run());
             MIXER_HIP(hipEventRecord(e1, b->s));
             MIXER_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            MIXER_HIP(hipEventElapsedTime(&ms, e0, e1));
-            t.push_back(ms);
+            MIXER_HIP(hipEventElapsedTime(&t[k], e0, e1));
+            t[k] /= (float)reps;
         }
-        std::sort(t.begin(), t.end());
-        *med = t.size() % 2 ? t[t.size() / 2] : 0.5f * (t[t.size() / 2 - 1] + t[t.size() / 2]);
+        std::sort(t, t + 3);
+        *med = t[1];
         return MIXER_OK;
     };
     mixer_batch_tuning win = start;

@@ -329,7 +329,7 @@ int mixer_batch_read_image(mixer_batch* b, int job, float* img, float* vmax, flo
 int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots);
 /* Autotuner for a batch: times runs of b on its current inputs (the last d_heaps given to
  * mixer_batch_run, which must have been called once) for each pack x tb_target candidate, each
- * as the median of reps individually timed runs after a warm-up run, and leaves b on the
+ * as the median over 3 blocks of reps back-to-back runs after a warm-up run, and leaves b on the
  * fastest, returned in *best. It moves off the tuning b had only for a candidate at least 2 %
  * faster, so noise cannot leave b slower than it started. */
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best);
