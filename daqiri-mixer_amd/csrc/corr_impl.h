@@ -11,6 +11,10 @@
 //           global -> LDS loads (global_load_lds_dwordx4) one stage ahead into two separately
 //           declared LDS buffers, so the barrier and LDS-latency bubble at each stage boundary
 //           is paid once per K blocks and no staging registers are needed.
+//   V = 6   the 16x16x64 tile shape (v_mfma_i32_16x16x64_i8, FOLD only): one instruction
+//           contracts two 32-sample blocks, so a stage is two blocks (one barrier per 64
+//           samples), staged through registers one stage ahead (an odd last block is paired with
+//           zeros).
 #pragma once
 #include "device_common.h"
 #include "kernels.h"
@@ -21,8 +25,35 @@ namespace mixer {
 
 template <int V, int NBLK>
 struct StageOf {
-    static constexpr int KB = (V == 1 || V == 3) ? 1 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
+    static constexpr int KB = (V == 1 || V == 3) ? 1 : V == 6 ? 2 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
 };
+
+// The MFMAs of one two-block stage in the 16x16x64 shape, tiles (RA, [A0,A1)) u (RB, [B0,B1)).
+template <int NE, int RA, int A0, int A1, int RB, int B0, int B1, int NA, int NBT>
+__device__ __forceinline__ void block_mfma16(const uint8_t* L, int lane, v4i (&ar)[NA][4],
+                                             v4i (&ai)[NA][4], v4i (&br)[NBT][4], v4i (&bi)[NBT][4]) {
+    constexpr int JLO = (RB >= 0 && B0 < A0) ? B0 : A0;
+    constexpr int JHI = (RB >= 0 && B1 > A1) ? B1 : A1;
+    const Frag fa[2] = {load_frag16<NE>(L, RA, 0, lane), load_frag16<NE>(L, RA, 1, lane)};
+    Frag fb[2] = {fa[0], fa[1]};
+    if constexpr (RB >= 0) {
+        fb[0] = load_frag16<NE>(L, RB, 0, lane);
+        fb[1] = load_frag16<NE>(L, RB, 1, lane);
+    }
+    static_for<JLO, JHI>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        constexpr bool inA = J >= A0 && J < A1;
+        constexpr bool inB = RB >= 0 && J >= B0 && J < B1;
+        if constexpr (inA || inB) {
+            Frag b[2];
+            if constexpr (J == RA) { b[0] = fa[0]; b[1] = fa[1]; }
+            else if constexpr (J == RB) { b[0] = fb[0]; b[1] = fb[1]; }
+            else { b[0] = load_frag16<NE>(L, J, 0, lane); b[1] = load_frag16<NE>(L, J, 1, lane); }
+            if constexpr (inA) tile_step16(fa, b, ar[J - A0], ai[J - A0]);
+            if constexpr (inB) tile_step16(fb, b, br[J - B0], bi[J - B0]);
+        }
+    });
+}
 
 // The MFMAs of one 32-sample block for tiles (RA, [A0,A1)) u (RB, [B0,B1)) from the LDS slab L
 // (NE elements, swizzled as lds_unit<NE>).
@@ -204,11 +235,105 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
 // starting at element block d0 (0 for a whole array; the diagonal triangles of the BLOCKED
 // decomposition use d0 = 8k); acc points at the channel's tile-packed accumulator (tile
 // t = I(I+1)/2 + J, 2048 ints).
+// FOLD wave program in the 16x16x64 shape (V = 6): two-block stages staged through registers
+// one stage ahead into two LDS stage buffers, one barrier per stage.
+template <int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
+__device__ __forceinline__ void fold_prog16(const int8_t* __restrict__ src, const XSrc& m,
+                                            int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
+                                            int n_kb, int accumulate, int ltid, bool chk,
+                                            unsigned& saw, int d0) {
+    constexpr int NE = NB * 32;
+    constexpr int NA = A1 - A0;
+    constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
+    const int lane = threadIdx.x & 63;
+    v4i ar[NA][4], ai[NA][4], br[NBT][4], bi[NBT][4];
+#pragma unroll
+    for (int j = 0; j < NA; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { ar[j][q] = (v4i){0, 0, 0, 0}; ai[j][q] = (v4i){0, 0, 0, 0}; }
+#pragma unroll
+    for (int j = 0; j < NBT; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { br[j][q] = (v4i){0, 0, 0, 0}; bi[j][q] = (v4i){0, 0, 0, 0}; }
+    using St = Stager<NE, NB, NTH>;
+    St s0, s1;  // the two blocks of the stage in flight
+    auto gload = [&](St& st, const int8_t* sl) {
+#pragma unroll
+        for (int i = 0; i < St::PER; ++i) {
+            const int u = ltid + i * NTH;
+            if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                st.r[i] = gld((const v4i*)(sl + m.off(p, d0 * 32 + b * 32 + (eg >> 1), eg & 1)));
+            }
+        }
+    };
+    auto check = [&](const St& st) {
+#pragma unroll
+        for (int i = 0; i < St::PER; ++i) {
+            const int u = ltid + i * NTH;
+            if ((St::CHUNKS % NTH == 0 || u < St::CHUNKS) && ((u >> 6) & 1))
+                saw |= has80(st.r[i].x) | has80(st.r[i].y) | has80(st.r[i].z) | has80(st.r[i].w);
+        }
+    };
+    // block slab L of the stage; zero: the stage's second block is past the end (zeros)
+    auto sstore = [&](const St& st, uint8_t* L, bool zero) {
+#pragma unroll
+        for (int i = 0; i < St::PER; ++i) {
+            const int u = ltid + i * NTH;
+            if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
+                const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                const int e = b * 32 + (eg >> 1);
+                *(v4i*)(L + (lds_unit<NE>(p, e, eg & 1) << 4)) = zero ? (v4i){0, 0, 0, 0} : st.r[i];
+            }
+        }
+    };
+    const int n_st = (n_kb + 1) / 2;
+    const int8_t* lastp = src + (size_t)(n_kb - 1) * m.tb;
+    gload(s0, src);
+    gload(s1, n_kb > 1 ? src + m.tb : lastp);
+    if (chk) { check(s0); if (n_kb > 1) check(s1); }
+    sstore(s0, lds0, false);
+    sstore(s1, lds0 + NE * 64, n_kb < 2);
+    __syncthreads();
+    for (int st = 0; st < n_st; ++st) {
+        uint8_t* cur = (st & 1) ? lds1 : lds0;
+        uint8_t* nxt = (st & 1) ? lds0 : lds1;
+        const int kb = 2 * (st + 1);
+        const bool more = st + 1 < n_st;
+        if (more) {
+            gload(s0, src + (size_t)kb * m.tb);
+            gload(s1, kb + 1 < n_kb ? src + (size_t)(kb + 1) * m.tb : lastp);
+        }
+        block_mfma16<NE, RA, A0, A1, RB, B0, B1, NA, NBT>(cur, lane, ar, ai, br, bi);
+        if (more) {
+            if (chk) { check(s0); if (kb + 1 < n_kb) check(s1); }
+            sstore(s0, nxt, false);
+            sstore(s1, nxt + NE * 64, kb + 1 >= n_kb);
+        }
+        __syncthreads();
+    }
+    const int IA = d0 + RA, IB = d0 + RB;
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+        int32_t* t = acc + ((size_t)IA * (IA + 1) / 2 + d0 + A0 + j) * 2048;
+        store_tile16(ar[j], t, lane, accumulate);
+        store_tile16(ai[j], t + 1024, lane, accumulate);
+    }
+    if constexpr (RB >= 0) {
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+            int32_t* t = acc + ((size_t)IB * (IB + 1) / 2 + d0 + B0 + j) * 2048;
+            store_tile16(br[j], t, lane, accumulate);
+            store_tile16(bi[j], t + 1024, lane, accumulate);
+        }
+    }
+}
+
 template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
-__device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const XSrc& m,
-                                          int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
-                                          int n_kb, int accumulate, int ltid, bool chk,
-                                          unsigned& saw, int d0) {
+__device__ __forceinline__ void fold_prog32(const int8_t* __restrict__ src, const XSrc& m,
+                                            int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
+                                            int n_kb, int accumulate, int ltid, bool chk,
+                                            unsigned& saw, int d0) {
     constexpr int NA = A1 - A0;
     constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
@@ -235,6 +360,19 @@ __device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const 
             store_tile(bi[j], t + 1024, lrow, kg, accumulate);
         }
     }
+}
+
+template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
+__device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const XSrc& m,
+                                          int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
+                                          int n_kb, int accumulate, int ltid, bool chk,
+                                          unsigned& saw, int d0) {
+    if constexpr (V == 6)
+        fold_prog16<NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid,
+                                                     chk, saw, d0);
+    else
+        fold_prog32<V, NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, acc, lds0, lds1, n_kb, accumulate,
+                                                        ltid, chk, saw, d0);
 }
 
 // Wave layout of the FOLD correlator. Row pair p = (p, NB-1-p) holds T_p = NB+1 tiles (the

@@ -162,4 +162,106 @@ __device__ __forceinline__ Frag load_frag(const uint8_t* lds, int B, int lrow, i
     return f;
 }
 
+// ------------------------------------------------------------------------------------------
+// The 16x16x64 tile shape. v_mfma_i32_16x16x64_i8 (checked on gfx950 with a single-instruction
+// probe): A lane l holds row l%16, k = 16*(l/16)..+15; B lane l holds column l%16 with the same k;
+// D lane l register r holds row 4*(l/16) + r, column l%16. One instruction contracts 64 samples,
+// two 32-sample blocks, so a stage holds two blocks [kb][plane][element][32] (each block's slab
+// swizzled as lds_unit<NE>) and lane l reads block (l/16)/2, 16-byte half (l/16)%2. A 32x32 tile
+// of the accumulator is four 16x16 sub-tiles s = 2*hr + hc (rows 16*hr.., columns 16*hc..).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ v4i mfma16(v4i a, v4i b, v4i c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+// Rows 16*h .. 16*h+15 of element block B for the 64 samples of a two-block stage at L.
+template <int NE>
+__device__ __forceinline__ Frag load_frag16(const uint8_t* L, int B, int h, int lane) {
+    const int e = B * 32 + 16 * h + (lane & 15), g = lane >> 4;
+    const uint8_t* blk = L + (g >> 1) * (NE * 64);
+    Frag f;
+    f.re = *(const v4i*)(blk + (lds_unit<NE>(0, e, g & 1) << 4));
+    f.im = *(const v4i*)(blk + (lds_unit<NE>(1, e, g & 1) << 4));
+    f.ng = neg8x16(f.im);
+    return f;
+}
+
+// One 32x32 tile over a 64-sample stage: rows a[hr], columns b[hc], the same four real products
+// as tile_step on each sub-tile.
+__device__ __forceinline__ void tile_step16(const Frag (&a)[2], const Frag (&b)[2], v4i (&vr)[4],
+                                            v4i (&vi)[4]) {
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr)
+#pragma unroll
+        for (int hc = 0; hc < 2; ++hc) {
+            const int s = 2 * hr + hc;
+            vr[s] = mfma16(a[hr].re, b[hc].re, vr[s]);
+            vi[s] = mfma16(a[hr].re, b[hc].ng, vi[s]);
+            vr[s] = mfma16(a[hr].ng, b[hc].ng, vr[s]);
+            vi[s] = mfma16(a[hr].im, b[hc].re, vi[s]);
+        }
+}
+
+// Half HC (columns 16*HC..) of one 32x32 tile over a 64-sample stage: rows a[hr], columns b.
+template <int HC>
+__device__ __forceinline__ void tile_half16(const Frag (&a)[2], const Frag& b, v4i (&vr)[4], v4i (&vi)[4]) {
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr) {
+        const int s = 2 * hr + HC;
+        vr[s] = mfma16(a[hr].re, b.re, vr[s]);
+        vi[s] = mfma16(a[hr].re, b.ng, vi[s]);
+        vr[s] = mfma16(a[hr].ng, b.ng, vr[s]);
+        vi[s] = mfma16(a[hr].im, b.re, vi[s]);
+    }
+}
+
+// The same on a tile held as one v16i (sub-tile s in registers 4s..4s+3).
+template <int S>
+__device__ __forceinline__ void mfma16_sub(v16i& acc, v4i a, v4i b) {
+    v4i c = {acc[4 * S], acc[4 * S + 1], acc[4 * S + 2], acc[4 * S + 3]};
+    c = mfma16(a, b, c);
+    acc[4 * S] = c.x;
+    acc[4 * S + 1] = c.y;
+    acc[4 * S + 2] = c.z;
+    acc[4 * S + 3] = c.w;
+}
+template <int HC>
+__device__ __forceinline__ void tile_half16v(const Frag (&a)[2], const Frag& b, v16i& vr, v16i& vi) {
+    mfma16_sub<HC>(vr, a[0].re, b.re);
+    mfma16_sub<HC>(vi, a[0].re, b.ng);
+    mfma16_sub<HC>(vr, a[0].ng, b.ng);
+    mfma16_sub<HC>(vi, a[0].im, b.re);
+    mfma16_sub<2 + HC>(vr, a[1].re, b.re);
+    mfma16_sub<2 + HC>(vi, a[1].re, b.ng);
+    mfma16_sub<2 + HC>(vr, a[1].ng, b.ng);
+    mfma16_sub<2 + HC>(vi, a[1].im, b.re);
+}
+__device__ __forceinline__ void store_tile16v(const v16i& x, int32_t* __restrict__ dst, int lane,
+                                              int accumulate) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int s = r >> 2, q = r & 3;
+        const int row = 16 * (s >> 1) + 4 * (lane >> 4) + q, col = 16 * (s & 1) + (lane & 15);
+        int32_t* p = dst + row * 32 + col;
+        int v = x[r];
+        if (accumulate) v += gld(p);
+        gst(p, v);
+    }
+}
+
+// Store one 32x32 tile held as four 16x16 sub-tiles to [32][32] row-major memory.
+__device__ __forceinline__ void store_tile16(const v4i (&x)[4], int32_t* __restrict__ dst, int lane,
+                                             int accumulate) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = 16 * (s >> 1) + 4 * (lane >> 4) + q, col = 16 * (s & 1) + (lane & 15);
+            int32_t* p = dst + row * 32 + col;
+            int v = x[s][q];
+            if (accumulate) v += gld(p);
+            gst(p, v);
+        }
+}
+
 }  // namespace mixer

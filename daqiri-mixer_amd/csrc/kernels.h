@@ -26,6 +26,7 @@ struct XcorrPlan {
     int prefetch;         // time-loop pipeline: 1 = registers one block ahead, 2/4 = LDS DMA stages
     int ct_ch;            // corner-turn channels per workgroup: 8, 16 or 32
     int fused;            // contexts: 1 = correlate straight from the heaps, 0 = via tiles
+    int mfma;             // MFMA tile shape: 32 (v_mfma_i32_32x32x32_i8) or 16 (16x16x64, FOLD)
     int n_tasks;          // BLOCKED only
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
     int n_quad;           // QUAD items: int2 (d0, -1) triangle or (r0, c0) rectangle
@@ -109,8 +110,9 @@ struct BatchQueues {
 hipError_t launch_batch_reset(hipStream_t s, int* counters);
 hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs);
 // fold items (arrays of <= 8 element blocks) / BLOCKED items (more); counters: 8 ints each
+// mfma: 32 = v_mfma_i32_32x32x32_i8 tiles, 16 = v_mfma_i32_16x16x64_i8 (fold items)
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              const BatchQueues& q, int* counters, int n_wgs);
+                              const BatchQueues& q, int* counters, int n_wgs, int mfma);
 hipError_t launch_batch_xcorr_blk(hipStream_t s, const BatchJobDev* jobs, const int4* items,
                                   const BatchQueues& q, int* counters, int n_wgs);
 hipError_t launch_batch_integrate(hipStream_t s, const BatchJobDev* jobs, int n_jobs, int n_wgs,

@@ -134,6 +134,97 @@ __device__ __forceinline__ void corr_wave_rt(const int8_t* __restrict__ src, con
     }
 }
 
+// The same program in the 16x16x64 tile shape: two-block (64-sample) stages, one barrier per
+// stage, an odd last block paired with zeros; the accumulator tiles as four 16x16 sub-tiles.
+__device__ __forceinline__ Frag load_frag16_rt(const uint8_t* L, int ne, int B, int h, int lane) {
+    const int e = B * 32 + 16 * h + (lane & 15), g = lane >> 4;
+    const uint8_t* blk = L + (g >> 1) * (ne * 64);
+    Frag f;
+    f.re = *(const v4i*)(blk + (lds_unit_rt(ne, 0, e, g & 1) << 4));
+    f.im = *(const v4i*)(blk + (lds_unit_rt(ne, 1, e, g & 1) << 4));
+    f.ng = neg8x16(f.im);
+    return f;
+}
+
+template <typename GB>
+__device__ __forceinline__ void corr_wave_rt16(const int8_t* __restrict__ src, const XSrc& m, int nblk,
+                                               GB gblk, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                               const WaveTiles& wt, bool active,
+                                               int32_t* __restrict__ acc, int ltid, int nth) {
+    const int lane = threadIdx.x & 63;
+    const int ne = nblk * 32;
+    v16i ar[kMaxT], ai[kMaxT];  // four 16x16 sub-tiles per tile
+#pragma unroll
+    for (int s = 0; s < kMaxT; ++s) { ar[s] = (v16i){0}; ai[s] = (v16i){0}; }
+    // Staging by direct global -> LDS loads (no staging registers): a stage is 2 blocks x 2 planes
+    // x nblk element blocks of 1 KiB windows; lane l of a window fills its 16-byte unit l (element
+    // l / 2, sample half (l & 1) ^ swizzle), so the layout is lds_unit_rt's. A block past the end
+    // is written as zeros.
+    const int el = lane >> 1, hsw = ((lane & 1) ^ ((el >> 2) ^ (el >> 3))) & 1;
+    const int nwin = 2 * nblk, wg = ltid >> 6, nw = nth >> 6;
+    auto dma = [&](uint8_t* L, int kb_first) {
+        if (ltid < 0) return;
+        for (int win = wg; win < 2 * nwin; win += nw) {
+            const int kbl = win / nwin, r = win % nwin, p = r / nblk, b = r % nblk;
+            const int kb = kb_first + kbl;
+            uint8_t* l = L + kbl * (ne * 64) + p * (ne * 32) + b * 1024;
+            if (kb < n_kb) {
+                const int8_t* g = src + (size_t)kb * m.tb + m.off(p, gblk(b) * 32 + el, hsw);
+                __builtin_amdgcn_global_load_lds((const void*)g, MIXER_LDS_PTR(l), 16, 0, 0);
+            } else {
+                *(v4i*)(l + lane * 16) = (v4i){0, 0, 0, 0};
+            }
+        }
+    };
+    const int n_st = (n_kb + 1) / 2;
+    dma(lds0, 0);
+    __syncthreads();  // waits for this wave's loads into LDS before the barrier
+    for (int sg = 0; sg < n_st; ++sg) {
+        uint8_t* cur = (sg & 1) ? lds1 : lds0;
+        uint8_t* nxt = (sg & 1) ? lds0 : lds1;
+        if (sg + 1 < n_st) dma(nxt, 2 * (sg + 1));
+        if (active) {
+            // row A's tiles, then row B's: one row's fragments live at a time, and each column
+            // fragment half is read just before its MFMAs
+            {
+                const Frag fa[2] = {load_frag16_rt(cur, ne, wt.ra, 0, lane), load_frag16_rt(cur, ne, wt.ra, 1, lane)};
+#pragma unroll
+                for (int s = 0; s < kMaxT; ++s) {
+                    if (s < wt.na) {
+                        tile_half16v<0>(fa, load_frag16_rt(cur, ne, wt.a0 + s, 0, lane), ar[s], ai[s]);
+                        tile_half16v<1>(fa, load_frag16_rt(cur, ne, wt.a0 + s, 1, lane), ar[s], ai[s]);
+                    }
+                    // keep the next tile's fragment reads below these MFMAs (register pressure)
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if (wt.nb) {
+                const Frag fb[2] = {load_frag16_rt(cur, ne, wt.rb, 0, lane), load_frag16_rt(cur, ne, wt.rb, 1, lane)};
+#pragma unroll
+                for (int s = 0; s < kMaxT; ++s) {
+                    if (s >= wt.na && s < wt.na + wt.nb) {
+                        tile_half16v<0>(fb, load_frag16_rt(cur, ne, wt.b0 + s - wt.na, 0, lane), ar[s], ai[s]);
+                        tile_half16v<1>(fb, load_frag16_rt(cur, ne, wt.b0 + s - wt.na, 1, lane), ar[s], ai[s]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int s = 0; s < kMaxT; ++s) {
+        if (s < wt.na + wt.nb) {
+            const int I = s < wt.na ? wt.ra_g : wt.rb_g;
+            const int J = s < wt.na ? wt.a0_g + s : wt.b0_g + s - wt.na;
+            int32_t* t = acc + (size_t)(I * (I + 1) / 2 + J) * 2048;
+            store_tile16v(ar[s], t, lane, 0);
+            store_tile16v(ai[s], t + 1024, lane, 0);
+        }
+    }
+}
+
 // Waves per channel of the folded layout in the persistent kernel: NB = 1 is one tile on one
 // wave; NB >= 2 splits each row pair (p, NB-1-p) over two waves (at most 5 tiles per wave).
 __device__ __host__ inline int fold_waves_rt(int nb) { return nb == 1 ? 1 : 2 * ((nb + 1) / 2); }
@@ -195,10 +286,14 @@ __device__ __forceinline__ int next_item(const BatchQueues& q, int* counters, in
 // and, once that queue is empty, from the other XCDs' queues in turn, until all are empty. A
 // queue holds whole channels, largest first, so the super-blocks that share a channel's element
 // blocks run together on one XCD and share its L2; the queues end with the short items.
+// SHAPE 32: v_mfma_i32_32x32x32_i8 tiles, one-block stages; 16: v_mfma_i32_16x16x64_i8, two-block
+// stages (twice the LDS).
+template <int SHAPE>
 __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restrict__ jobs,
                                                      const int4* __restrict__ items, BatchQueues q,
                                                      int* __restrict__ counters) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kBatchLds];
+    constexpr int KB = SHAPE == 16 ? 2 : 1;  // blocks per stage
+    __shared__ __attribute__((aligned(16))) uint8_t lds[KB * kBatchLds];
     __shared__ int item_sh;
     const int w8 = threadIdx.x >> 6;
     const int home = blockIdx.x & 7;
@@ -220,7 +315,7 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
             bool active = false;
             const int wpc = fold_waves_rt(nb), G = min(8 / wpc, J.ch_per_item);
             const int g = w8 / wpc;
-            const int nblk = nb, buf = nb * 32 * 64, nth = 64 * wpc;
+            const int nblk = nb, buf = KB * nb * 32 * 64, nth = 64 * wpc;
             const int ltid = g < G ? (int)(threadIdx.x % nth) : -1;
             int c = w.y + (g < G ? g : 0);
             const bool have = g < G && c < J.n_chan;
@@ -234,8 +329,12 @@ __global__ __launch_bounds__(512) void k_batch_xcorr(const BatchJobDev* __restri
             };
             const int8_t* src = m.base + (size_t)c * m.chan + (size_t)kb0 * m.tb;
             int32_t* a = J.acc + ((size_t)slot * J.n_chan + c) * NT * 2048;
-            corr_wave_rt(src, m, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
-                         ltid, nth);
+            if constexpr (SHAPE == 16)
+                corr_wave_rt16(src, m, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
+                               ltid, nth);
+            else
+                corr_wave_rt(src, m, nblk, gblk, l0, l0 + buf, kb1 - kb0, wt, active && have, a,
+                             ltid, nth);
         }
         __syncthreads();  // LDS reuse by the next item
     }
@@ -307,8 +406,11 @@ hipError_t launch_batch_corner_turn(hipStream_t s, const BatchJobDev* jobs, int 
 }
 
 hipError_t launch_batch_xcorr(hipStream_t s, const BatchJobDev* jobs, const int4* items,
-                              const BatchQueues& q, int* counters, int n_wgs) {
-    hipLaunchKernelGGL(k_batch_xcorr, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
+                              const BatchQueues& q, int* counters, int n_wgs, int mfma) {
+    if (mfma == 16)
+        hipLaunchKernelGGL(k_batch_xcorr<16>, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
+    else
+        hipLaunchKernelGGL(k_batch_xcorr<32>, dim3(n_wgs), dim3(512), 0, s, jobs, items, q, counters);
     return hipGetLastError();
 }
 

@@ -1,5 +1,6 @@
 // kernels_corr.hip -- dispatch of the correlator launch plan to the kernel variants
-// (corr_impl.h; instantiated per pipeline variant in corr_{fold,blk}_pf{1,2,3,4}.hip).
+// (corr_impl.h; instantiated per pipeline variant in corr_{fold,blk}_pf{1,2,3,4}.hip and, for the
+// 16x16x64 tile shape, corr_fold_m16.hip).
 #include "kernels.h"
 
 namespace mixer {
@@ -18,6 +19,8 @@ MIXER_DECL(blk, 3)
 MIXER_DECL(blk, 4)
 #undef MIXER_DECL
 
+hipError_t launch_fold_m16(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
+                           uint32_t* wrap, int C, int n_tb, int accumulate);
 hipError_t launch_quad_1(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
                          uint32_t* wrap, int C, int n_tb, int accumulate);
 hipError_t launch_quad_3(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
@@ -31,6 +34,7 @@ hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const XSrc& src, int3
         return hipErrorInvalidValue;
     }
     const bool fold = p.kind == XcorrPlan::FOLD;
+    if (fold && p.mfma == 16) return launch_fold_m16(s, p, src, acc, wrap, C, n_tb, accumulate);
     switch (p.prefetch) {
     case 1: return fold ? launch_fold_1(s, p, src, acc, wrap, C, n_tb, accumulate)
                         : launch_blk_1(s, p, src, acc, wrap, C, n_tb, accumulate);

@@ -32,7 +32,7 @@ std::map<int, mixer_tuning> g_tuning;  // n_elem -> tuning (0 fields = automatic
 mixer_tuning tuning_for(int n_elem) {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     auto it = g_tuning.find(n_elem);
-    if (it == g_tuning.end()) return mixer_tuning{0, 0, 0, 0};
+    if (it == g_tuning.end()) return mixer_tuning{0, 0, 0, 0, 0};
     return it->second;
 }
 
@@ -81,6 +81,7 @@ int plan_kind(const mixer_array_cfg& c) {
     // on the per-channel BLOCKED layout: their QUAD rectangles move twice the bytes through L2
     // per tile and measured slower (1024 x 256 x 4096: 2.91 vs 2.80 ms). The direct-to-LDS
     // variants keep the per-channel layouts.
+    if (t.mfma == 16 && nb <= 8) return XcorrPlan::FOLD;  // the 16x16x64 shape runs on FOLD
     if (plan_fused(c) && (nb == 4 || nb == 8) && (pf == 1 || pf == 3)) return XcorrPlan::QUAD;
     return nb <= 8 ? XcorrPlan::FOLD : XcorrPlan::BLOCKED;
 }
@@ -149,6 +150,7 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
     p->fused = plan_fused(c) ? 1 : 0;
     p->prefetch = t.prefetch > 0 ? t.prefetch : 3;
     p->ct_ch = t.ct_chan > 0 ? t.ct_chan : 8;
+    p->mfma = (t.mfma == 16 && nb <= 8) ? 16 : 32;
     p->n_tasks = 0;
     p->d_tasks = nullptr;
     {
@@ -405,6 +407,7 @@ int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
     if (t->prefetch < 0 || t->prefetch > 4) return MIXER_ERR_ARG;
     if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
     if (t->staging < 0 || t->staging > 2) return MIXER_ERR_ARG;
+    if (t->mfma != 0 && t->mfma != 16 && t->mfma != 32) return MIXER_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     g_tuning[n_elem] = *t;
     return MIXER_OK;
@@ -420,6 +423,7 @@ int mixer_get_tuning(const mixer_array_cfg* c, mixer_tuning* t) {
     t->prefetch = p.prefetch;
     t->ct_chan = p.ct_ch;
     t->staging = p.fused ? 1 : 2;
+    t->mfma = p.mfma;
     return MIXER_OK;
 }
 

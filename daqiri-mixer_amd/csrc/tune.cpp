@@ -92,15 +92,19 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
 
     const mixer_tuning saved = tuning_for(c.n_elem);
     const bool had = has_tuning(c.n_elem);
-    mixer_tuning t{0, 0, 32, 0}, win{0, 0, 32, 0};
+    mixer_tuning t{0, 0, 32, 0, 32}, win{0, 0, 32, 0, 32};
     float best_ms = 1e30f;
     int st = MIXER_OK;
+    // the 32x32x32 shape with each pipeline variant, then (arrays of <= 8 element blocks) the
+    // 16x16x64 shape, whose two-block stages have a pipeline of their own
+    const int n_pf = nb <= 8 ? 5 : 4;
     for (int staging : {1, 2}) {
         for (int slots : {1, 2, 4, 8}) {
             if (slots > n_tb) continue;
-            for (int pf : {1, 2, 3, 4}) {
+            for (int k = 0; k < n_pf; ++k) {
                 t.slots = slots;
-                t.prefetch = pf;
+                t.prefetch = k < 4 ? k + 1 : 0;
+                t.mfma = k < 4 ? 32 : 16;
                 t.staging = staging;
                 if ((st = mixer_set_tuning(c.n_elem, &t))) break;
                 float ms = 0;
@@ -123,7 +127,7 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
         }
     }
     if (st) {  // leave the table as it was
-        const mixer_tuning automatic{0, 0, 0, 0};
+        const mixer_tuning automatic{0, 0, 0, 0, 0};
         (void)mixer_set_tuning(c.n_elem, had ? &saved : &automatic);
         return st;
     }

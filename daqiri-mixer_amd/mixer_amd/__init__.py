@@ -50,20 +50,21 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("staging", C.c_int32)]
+    _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("staging", C.c_int32),
+                ("mfma", C.c_int32)]
 
     def as_dict(self):
         return {"slots": self.slots, "prefetch": self.prefetch, "ct_chan": self.ct_chan,
-                "staging": self.staging}
+                "staging": self.staging, "mfma": self.mfma}
 
 
 class BatchTuning(C.Structure):
     _fields_ = [("pack", C.c_int32), ("tb_target", C.c_int32), ("use_graph", C.c_int32),
-                ("staging", C.c_int32)]
+                ("staging", C.c_int32), ("mfma", C.c_int32)]
 
     def as_dict(self):
         return {"pack": self.pack, "tb_target": self.tb_target, "use_graph": self.use_graph,
-                "staging": self.staging}
+                "staging": self.staging, "mfma": self.mfma}
 
 
 class BatchInfo(C.Structure):
@@ -158,6 +159,8 @@ SIGNATURES = [
     ("mixer_batch_read_image", I32, [P, I32, P, P, P]),
     ("mixer_batch_acc", I32, [P, I32, P, P]),
     ("mixer_batch_autotune", I32, [P, I32, P]),
+    ("mixer_batch_set_tuning", I32, [P, P]),
+    ("mixer_batch_get_tuning", I32, [P, P]),
     ("mixer_fill_random", I32, [P, P, SZ, U64]),
     ("mixer_server_create", I32, [I32, U64, I32, P]),
     ("mixer_server_destroy", I32, [P]),
@@ -375,8 +378,9 @@ def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):
           "mixer_xcorr_plain")
 
 
-def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0, staging=0):
-    t = Tuning(slots, prefetch, ct_chan, staging)
+def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0, staging=0, mfma=0):
+    """mfma: 32 = v_mfma_i32_32x32x32_i8 tiles, 16 = v_mfma_i32_16x16x64_i8 (<= 256 elements), 0 = auto."""
+    t = Tuning(slots, prefetch, ct_chan, staging, mfma)
     check(lib().mixer_set_tuning(n_elem, C.byref(t)), "mixer_set_tuning")
 
 
@@ -581,6 +585,14 @@ class Batch:
         out = np.empty((c.n_chan, 2, c.n_elem, c.n_elem), np.float32)
         check(lib().mixer_batch_dump(self.h, job, ptr(out)), "mixer_batch_dump")
         return out
+
+    def set_tuning(self, t: BatchTuning):
+        check(lib().mixer_batch_set_tuning(self.h, C.byref(t)), "mixer_batch_set_tuning")
+
+    def tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        check(lib().mixer_batch_get_tuning(self.h, C.byref(t)), "mixer_batch_get_tuning")
+        return t
 
     def acc(self, job: int):
         """(device address, time slots) of job's int32 accumulator of the last run."""

@@ -160,8 +160,13 @@ int mixer_image_noalloc(void* stream, const mixer_array_cfg* cfg, const int32_t*
  *   staging   how contexts feed the correlator: 1 = straight from the heaps (the corner-turn
  *             is fused into the correlator's loads, mixer_xcorr_heaps), 2 = a separate
  *             corner-turn pass into tiles (mixer_corner_turn + mixer_xcorr)
- * 0 in a field means automatic: staging 1 with prefetch 3 (the measured fastest choice on
- * MI355X for both the fold and the blocked correlator). The table is process-wide. slots changes the accumulator
+ *   mfma      MFMA tile shape: 32 = v_mfma_i32_32x32x32_i8 (32x32 tiles, one 32-sample block per
+ *             instruction), 16 = v_mfma_i32_16x16x64_i8 (each 32x32 tile as four 16x16 tiles,
+ *             two 32-sample blocks per instruction and per barrier; arrays of <= 256 elements,
+ *             on the FOLD layout, prefetch ignored)
+ * 0 in a field means automatic: staging 1 with prefetch 3 and the 32x32x32 shape (the measured
+ * fastest choice on MI355X for both the fold and the blocked correlator). The table is
+ * process-wide. slots changes the accumulator
  * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
  * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
  * src/cgemm.cuh:18-24). */
@@ -170,13 +175,15 @@ typedef struct mixer_tuning {
     int32_t prefetch;
     int32_t ct_chan;
     int32_t staging;
+    int32_t mfma;
 } mixer_tuning;
 int mixer_set_tuning(int n_elem, const mixer_tuning* t);
 int mixer_get_tuning(const mixer_array_cfg* cfg, mixer_tuning* t); /* the plan in effect */
 /* Autotuner: times (corner-turn +) correlate + integrate + image of one integration of cfg on
- * synthetic device data for every (staging, slots, prefetch) candidate, then every ct_chan if
- * the separate corner-turn won, on device `device`; stores the fastest in the tuning table and
- * returns it in *best. */
+ * synthetic device data for every (staging, slots, prefetch) candidate in the 32x32x32 shape and,
+ * for arrays of <= 256 elements, every (staging, slots) candidate in the 16x16x64 shape, then
+ * every ct_chan if the separate corner-turn won, on device `device`; stores the fastest in the
+ * tuning table and returns it in *best. */
 int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuning* best);
 
 /* Plain per-channel-launch form (baseline): one corner-turn launch and one full-square GEMM
@@ -293,13 +300,16 @@ void* mixer_compute_stream(mixer_ctx* ctx);
  *   staging    1 = the correlator stages straight from the heaps (no corner-turn launch),
  *              2 = a separate corner-turn launch into tiles; 0 = automatic: 1 (the 64-array
  *              batch runs 108 ms fused, 121 ms with corner-turned tiles above 256 elements;
- *              fixed at creation) */
+ *              fixed at creation)
+ *   mfma       MFMA tile shape of the fold correlator (arrays up to 256 elements): 32 =
+ *              v_mfma_i32_32x32x32_i8, 16 = v_mfma_i32_16x16x64_i8 (two-block stages); 0 = 32 */
 typedef struct mixer_batch mixer_batch;
 typedef struct mixer_batch_tuning {
     int32_t pack;
     int32_t tb_target;
     int32_t use_graph;
     int32_t staging;
+    int32_t mfma;
 } mixer_batch_tuning;
 typedef struct mixer_batch_info {
     int32_t n_jobs;
@@ -328,11 +338,16 @@ int mixer_batch_dump(mixer_batch* b, int job, float* host_v);
 int mixer_batch_read_image(mixer_batch* b, int job, float* img, float* vmax, float* vmin);
 int mixer_batch_acc(mixer_batch* b, int job, const int32_t** d_acc, int* n_slots);
 /* Autotuner for a batch: times runs of b on its current inputs (the last d_heaps given to
- * mixer_batch_run, which must have been called once) for each pack x tb_target candidate, each
+ * mixer_batch_run, which must have been called once) for each mfma x pack x tb_target candidate, each
  * as the median over 3 blocks of reps back-to-back runs after a warm-up run, and leaves b on the
  * fastest, returned in *best. It moves off the tuning b had only for a candidate at least 2 %
- * faster, so noise cannot leave b slower than it started. */
+ * faster, confirmed by timing the two alternately once the search is over (the GPU's clock
+ * drifts during a search of seconds), so noise cannot leave b slower than it started. */
 int mixer_batch_autotune(mixer_batch* b, int reps, mixer_batch_tuning* best);
+/* Re-plan b under tuning t (its staging field is ignored: fixed at creation; the work list and
+ * graph are rebuilt), or read the tuning in effect. */
+int mixer_batch_set_tuning(mixer_batch* b, const mixer_batch_tuning* t);
+int mixer_batch_get_tuning(mixer_batch* b, mixer_batch_tuning* t);
 /* Fills n bytes of device memory with seeded uniform int8 values in [-127, 127]. */
 int mixer_fill_random(void* stream, void* d_ptr, size_t bytes, uint64_t seed);
 

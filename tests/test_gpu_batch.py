@@ -58,12 +58,15 @@ def check_job(b, j, cfg, bb, n_tb, images=True):
         assert np.array_equal(vmin, img.reshape(C, -1).min(1))
 
 
-@pytest.mark.parametrize("pack,use_graph,staging", [(0, 1, 0), (1, 1, 0), (0, 0, 0), (0, 1, 1), (1, 1, 2)])
-def test_batch_uneven_jobs_bit_exact(pack, use_graph, staging):
-    log(f"batch pack={pack} graph={use_graph} staging={staging} jobs={JOBS}")
+@pytest.mark.parametrize("pack,use_graph,staging,mfma", [(0, 1, 0, 0), (1, 1, 0, 0), (0, 0, 0, 0), (0, 1, 1, 0),
+                                                       (1, 1, 2, 0), (0, 1, 0, 16), (1, 1, 2, 16)])
+def test_batch_uneven_jobs_bit_exact(pack, use_graph, staging, mfma):
+    """Every job of a batch, in either MFMA tile shape of the fold correlator, bit-exact over
+    three graph replays with new data and a re-pointed job table."""
+    log(f"batch pack={pack} graph={use_graph} staging={staging} mfma={mfma} jobs={JOBS}")
     cfgs = [cfg_for(*j) for j in JOBS]
     t = M.batch_default_tuning()
-    t.pack, t.use_graph, t.staging = pack, use_graph, staging
+    t.pack, t.use_graph, t.staging, t.mfma = pack, use_graph, staging, mfma
     with M.Batch(cfgs, tuning=t) as b:
         info = b.info()
         assert info.n_jobs == len(JOBS) and info.graph == use_graph and info.n_items > 0
@@ -104,9 +107,20 @@ def test_batch_time_split_and_autotune():
         best = b.autotune(reps=3)
         log(f"  autotune -> {best.as_dict()}")
         # a candidate, or the starting tuning when no candidate beat it by 2 %
-        assert best.pack in (0, 1) and best.tb_target in (8, 64, 128, 256, 512)
+        assert best.pack in (0, 1) and best.tb_target in (8, 64, 128, 256, 512) and best.mfma in (0, 16, 32)
         if best.tb_target == 8:
             assert best.pack == t.pack
+        b.run(bufs)
+        b.sync()
+        for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):
+            check_job(b, j, cfg, bb, jb[2], images=False)
+        assert b.tuning().as_dict() == best.as_dict()
+        # re-planned under another tuning (the other MFMA shape), the batch stays exact
+        t2 = M.batch_default_tuning()
+        t2.tb_target, t2.mfma = 64, 16 if best.mfma != 16 else 32
+        b.set_tuning(t2)
+        got = b.tuning()
+        assert (got.tb_target, got.mfma) == (64, t2.mfma)
         b.run(bufs)
         b.sync()
         for j, (cfg, bb, jb) in enumerate(zip(cfgs, bbs, jobs)):

@@ -365,24 +365,26 @@ def test_every_launch_variant_bit_exact(N, C):
                     vre, vim = M.unpack_acc(d_acc.download(np.int32, (M.acc_bytes(cfg) // 4,)), cfg)
                     assert np.array_equal(vre, ore) and np.array_equal(vim, oim), (slots, pf, ct)
     finally:
-        M.lib().mixer_set_tuning(N, M.C.byref(M.Tuning(0, 0, 0, 0)))
+        M.lib().mixer_set_tuning(N, M.C.byref(M.Tuning(0, 0, 0, 0, 0)))
 
 
 def test_autotune_sets_a_valid_plan():
     cfg = cfg_for(128, 16, 1024)
     try:
         best = M.autotune(cfg)
-        assert best.slots in (1, 2, 4, 8) and best.prefetch in (1, 2, 3, 4) and best.ct_chan in (8, 16, 32)
-        assert best.staging in (1, 2)
+        log(f"autotune 128 x 16 x 1024 -> {best.as_dict()}")
+        assert best.slots in (1, 2, 4, 8) and best.ct_chan in (8, 16, 32) and best.staging in (1, 2)
+        assert (best.mfma, best.prefetch) in [(32, 1), (32, 2), (32, 3), (32, 4), (16, 0)]
         t = M.get_tuning(cfg)
-        assert (t.slots, t.prefetch, t.ct_chan) == (best.slots, best.prefetch, best.ct_chan)
+        assert (t.slots, t.ct_chan, t.mfma) == (best.slots, best.ct_chan, best.mfma)
+        assert best.mfma == 16 or t.prefetch == best.prefetch
         bb = rand_heaps(32, 128, 16)
         c2, _, acc = gpu_v(cfg, bb, 32)
         vre, vim = M.unpack_acc(acc, c2)
         ore, oim = O.v_full_from_heaps(bb, 32, 128, 16, 32)
         assert np.array_equal(vre, ore) and np.array_equal(vim, oim)
     finally:
-        M.lib().mixer_set_tuning(128, M.C.byref(M.Tuning(0, 0, 0, 0)))
+        M.lib().mixer_set_tuning(128, M.C.byref(M.Tuning(0, 0, 0, 0, 0)))
 
 
 # ---------------------------------------------------------------- fused corner-turn + correlate
@@ -415,6 +417,63 @@ def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
                 assert np.array_equal(vre, ore) and np.array_equal(vim, oim), (slots, pf)
     finally:
         M.set_tuning(N)
+
+
+# ---------------------------------------------------------------- the 16x16x64 tile shape
+@pytest.mark.parametrize("N,C,n_tb,parts", [
+    (256, 5, 8, 1), (256, 3, 7, 1), (256, 6, 12, 3), (224, 2, 5, 1), (192, 3, 6, 2), (160, 3, 5, 1),
+    (128, 9, 6, 2), (96, 2, 3, 1), (64, 9, 4, 1), (32, 3, 3, 1), (32, 1, 1, 1),
+])
+def test_mfma16_shape_bit_exact(N, C, n_tb, parts):
+    """The FOLD correlator in the v_mfma_i32_16x16x64_i8 shape (two 32-sample blocks per
+    instruction, an odd last block paired with zeros) gives the CPU correlator's V straight from
+    the heaps and from corner-turned tiles, for every time split and accumulated V-updates."""
+    log(f"mfma16 N={N} C={C} n_tb={n_tb} parts={parts}")
+    bb = rand_heaps(n_tb, N, C)
+    ore, oim = O.v_full_from_heaps(bb, n_tb, N, C, 32)
+    per = n_tb // parts
+    try:
+        for slots in (1, 2, 3):
+            if slots > per:
+                continue
+            M.set_tuning(N, slots=slots, mfma=16)
+            cfg = cfg_for(N, C, 32 * per)
+            assert M.get_tuning(cfg).mfma == 16 and M.acc_slots(cfg) == slots
+            d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+            d_acc2 = M.DeviceBuffer(M.acc_bytes(cfg))
+            d_t = M.DeviceBuffer(M.tiles_bytes(cfg, per))
+            for p in range(parts):
+                d_bb = M.DeviceBuffer.from_array(np.ascontiguousarray(bb[p * per:(p + 1) * per]))
+                M.xcorr_heaps(cfg, d_bb, per, d_acc, accumulate=p > 0)
+                M.corner_turn(cfg, d_bb, per, d_t)
+                M.xcorr(cfg, d_t, per, d_acc2, accumulate=p > 0)
+            for acc in (d_acc, d_acc2):
+                vre, vim = M.unpack_acc(acc.download(np.int32, (M.acc_bytes(cfg) // 4,)), cfg)
+                assert np.array_equal(vre, ore) and np.array_equal(vim, oim), slots
+    finally:
+        M.set_tuning(N)
+
+
+def test_mfma16_shape_counts_minus128_and_runs_the_engine():
+    """The 16x16x64 shape counts imaginary -128 like the other variants, and a context on it
+    dumps the exact V and its image."""
+    try:
+        M.set_tuning(256, mfma=16)
+        _check_minus128(256, 4)
+        cfg = cfg_for(256, 8, 32 * 6)
+        bb = rand_heaps(6, 256, 8)
+        d = M.DeviceBuffer.from_array(bb)
+        with M.Engine(cfg) as e:
+            e.token_wait(e.submit_device(d, 6))
+            e.sync()
+            v, idx = e.dump()
+            img, _, _, _ = e.read_image()
+        r, i = O.v_full_from_heaps(bb, 6, 256, 8, 32)
+        assert idx == 1 and np.array_equal(v, np.stack([r, i], 1).astype(np.float32))
+        o = O.image_from_v(r, i, cfg.grid_x, cfg.ng)[0]
+        assert np.abs(img - o).max() / np.abs(o).max() <= 1e-5
+    finally:
+        M.set_tuning(256)
 
 
 @pytest.mark.parametrize("N,C,pf", [(256, 4, 1), (256, 4, 3), (256, 4, 4), (96, 3, 2), (160, 2, 3), (512, 2, 1),
