@@ -25,15 +25,27 @@ struct QuadLds {
     static constexpr int BYTES = 4 * CS;
 };
 
-// Staging of one 32-sample block of NBLK element blocks for 4 channels: chunk u (16 B) is
-// element u / 16, plane (u / 8) % 2, channel (u / 2) % 4, half u % 2, so 8 consecutive chunks
-// are one contiguous 128-byte line and a wave's load covers 8 whole lines.
+// Staging of one 32-sample block of NBLK element blocks for 4 channels: thread t stages plane
+// t / 256 (waves 0-3 the real plane, waves 4-7 the imaginary one), and its chunk i (16 B) is
+// k = t % 256 + 256 i: line g = k / 8 (element e(g): lines 2j, 2j+1 of each group of 8 are elements
+// j and j + 4), channel (k / 2) % 4, half k % 2. 8 consecutive threads load one contiguous
+// 128-byte line and a wave's load covers 8 whole lines; in LDS the lines of elements j and j + 4
+// fall in disjoint halves of the 256-byte bank row (channel stride = 32 B mod 256), so the 16
+// lanes of each store phase hit distinct banks. Only the imaginary waves test for -128, one wave
+// per SIMD, with every lane busy.
 template <int NBLK>
 struct QuadStager {
     static constexpr int NE = NBLK * 32;
     static constexpr int CHUNKS = NE * 16;
     static constexpr int PER = CHUNKS / 512;
     v4i r[PER];
+    __device__ __forceinline__ static void chunk(int i, int& el, int& p, int& ch, int& h) {
+        const int t = threadIdx.x, k = (t & 255) + 256 * i, g = k >> 3;
+        p = t >> 8;
+        el = (g & ~7) | ((g & 1) << 2) | ((g >> 1) & 3);
+        ch = (k >> 1) & 3;
+        h = k & 1;
+    }
     // Buffer loads: the block's base in a scalar descriptor, one 32-bit offset register per load
     // (the same offsets for every block, so both register sets share them), where a global load
     // would keep a 64-bit address per load.
@@ -42,8 +54,8 @@ struct QuadStager {
         const __amdgpu_buffer_rsrc_t rs = block_rsrc(sl);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int u = threadIdx.x + i * 512;
-            const int el = u >> 4, p = (u >> 3) & 1, ch = (u >> 1) & 3, h = u & 1;
+            int el, p, ch, h;
+            chunk(i, el, p, ch, h);
             const int c = q4 + ch < C ? q4 + ch : C - 1;  // ragged last quad: re-read a valid channel
             const uint32_t o = m.off32(p, gblk(el >> 5) * 32 + (el & 31), h) + (uint32_t)c * (uint32_t)m.chan;
             r[i] = bload(rs, o);
@@ -52,20 +64,24 @@ struct QuadStager {
     __device__ __forceinline__ void store(uint8_t* L) const {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int u = threadIdx.x + i * 512;
-            const int el = u >> 4, p = (u >> 3) & 1, ch = (u >> 1) & 3, h = u & 1;
+            int el, p, ch, h;
+            chunk(i, el, p, ch, h);
             const int unit = 2 * el + (h ^ (((el >> 2) ^ (el >> 3)) & 1));
             *(v4i*)(L + ch * QuadLds<NE>::CS + p * QuadLds<NE>::PS + unit * 16) = r[i];
         }
     }
+    // Nonzero if an imaginary byte staged by this thread is -128 (three operations per register).
     __device__ __forceinline__ unsigned saw80() const {
         unsigned s = 0;
+        if (threadIdx.x < 256) return 0;  // real-plane waves (uniform per wave)
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int u = threadIdx.x + i * 512;
-            if ((u >> 3) & 1) s |= has80(r[i].x) | has80(r[i].y) | has80(r[i].z) | has80(r[i].w);
+            s = acc80(s, r[i].x);
+            s = acc80(s, r[i].y);
+            s = acc80(s, r[i].z);
+            s = acc80(s, r[i].w);
         }
-        return s;
+        return s & 0x80808080u;
     }
 };
 
@@ -132,16 +148,16 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
     if constexpr (V == 1) {
         St st;
         st.load(src, m, gblk, q4, C);
-        if (chk) saw |= st.saw80();
         st.store(lds0);
+        if (chk) saw |= st.saw80();
         __syncthreads();
         for (int kb = 0; kb < n_kb; ++kb) {
             const bool more = kb + 1 < n_kb;
             if (more) st.load(src + (size_t)(kb + 1) * m.tb, m, gblk, q4, C);
             compute((kb & 1) ? lds1 : lds0);
             if (more) {
-                if (chk) saw |= st.saw80();
                 st.store((kb & 1) ? lds0 : lds1);
+                if (chk) saw |= st.saw80();  // overlaps the stores' LDS latency
             }
             __syncthreads();
         }
@@ -153,8 +169,8 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
         St s0, s1;
         s0.load(src, m, gblk, q4, C);
         s1.load(src + (size_t)(n_kb > 1 ? 1 : 0) * m.tb, m, gblk, q4, C);
-        if (chk) saw |= s0.saw80();
         s0.store(lds0);
+        if (chk) saw |= s0.saw80();
         __syncthreads();
         const int8_t* nx = src + 2 * m.tb;
         const int8_t* last = src + (size_t)(n_kb - 1) * m.tb;
@@ -162,13 +178,13 @@ __device__ __forceinline__ void quad_wave(const int8_t* __restrict__ src, const 
         for (; kb + 2 <= n_kb; kb += 2) {
             s0.load(kb + 2 < n_kb ? nx : last, m, gblk, q4, C);
             compute(lds0);
-            if (chk) saw |= s1.saw80();
             s1.store(lds1);
+            if (chk) saw |= s1.saw80();  // after the stores: overlaps their LDS latency
             __syncthreads();
             s1.load(kb + 3 < n_kb ? nx + m.tb : last, m, gblk, q4, C);
             compute(lds1);
-            if (chk) saw |= s0.saw80();
             s0.store(lds0);
+            if (chk) saw |= s0.saw80();
             __syncthreads();
             nx += 2 * m.tb;
         }

@@ -29,6 +29,13 @@ __device__ __forceinline__ unsigned has80(int w) {
     unsigned t = (unsigned)w ^ 0x80808080u;
     return (t - 0x01010101u) & ~t & 0x80808080u;
 }
+// The same test folded into an accumulator: a byte of w equal to 0x80 leaves its bit 7 set in
+// ((w ^ 0x80808080) - 0x01010101) & w (the zero-byte test on w ^ 0x80808080, using
+// ~(w ^ 0x80808080) & 0x80808080 = w & 0x80808080). Three operations (xor, add, and-or); the
+// other bits are don't-cares, so the caller masks the accumulator with 0x80808080 once.
+__device__ __forceinline__ unsigned acc80(unsigned s, int w) {
+    return s | ((((unsigned)w ^ 0x80808080u) + 0xFEFEFEFFu) & (unsigned)w);
+}
 
 // MFMA lane layout (checked on gfx950 hardware with a single-instruction probe):
 //   v_mfma_i32_32x32x32_i8: A lane l holds row l%32, k = 16*(l/32)..+15 (16 bytes);
