@@ -7,6 +7,7 @@
 //   V = 1   one 32-sample block per barrier, staged global -> registers -> LDS one block ahead;
 //   V = 3   the same with two register sets, loads issued two blocks ahead (more bytes in
 //           flight per CU for the latency-bound gather straight from the heaps);
+//   V = 5   three register sets, loads issued three blocks ahead;
 //   V = 2,4 K = V blocks per barrier (BLOCKED: K = min(V, 3)), staged by direct
 //           global -> LDS loads (global_load_lds_dwordx4) one stage ahead into two separately
 //           declared LDS buffers, so the barrier and LDS-latency bubble at each stage boundary
@@ -25,7 +26,7 @@ namespace mixer {
 
 template <int V, int NBLK>
 struct StageOf {
-    static constexpr int KB = (V == 1 || V == 3) ? 1 : V == 6 ? 2 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
+    static constexpr int KB = (V == 1 || V == 3 || V == 5) ? 1 : V == 6 ? 2 : (NBLK > 8 ? (V < 3 ? V : 3) : V);
 };
 
 // The MFMAs of one two-block stage in the 16x16x64 shape, tiles (RA, [A0,A1)) u (RB, [B0,B1)).
@@ -147,6 +148,63 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
             block_mfma<NE, RA, A0, A1, RB, B0, B1>(lds0, lrow, kg, ar, ai, br, bi);
             __syncthreads();
         }
+    } else if constexpr (V == 5) {
+        // three register sets, loads issued three blocks ahead: step j of a six-step round
+        // computes block kb+j from LDS buffer j % 2, loads block kb+j+3 into set j % 3 (whose
+        // block kb+j was staged one step earlier) and stages set (j+1) % 3 (block kb+j+1, loaded
+        // two steps earlier) into the other buffer. Six steps (three sets x two buffers) keep
+        // every set in fixed registers; loads past the end re-read the last block (never
+        // computed), as in V = 3.
+        auto ident = [](int b) { return b; };
+        using St = Stager<NE, NBLK, NTH>;
+        St s0, s1, s2;
+        auto gload = [&](St& st, const int8_t* sl) {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if (St::CHUNKS % NTH == 0 || u < St::CHUNKS) {
+                    const int b = u >> 7, w = u & 127, p = w >> 6, eg = w & 63;
+                    st.r[i] = gld((const v4i*)(sl + m.off(p, gblk(b) * 32 + (eg >> 1), eg & 1)));
+                }
+            }
+        };
+        auto check = [&](const St& st) {
+#pragma unroll
+            for (int i = 0; i < St::PER; ++i) {
+                const int u = ltid + i * NTH;
+                if ((St::CHUNKS % NTH == 0 || u < St::CHUNKS) && ((u >> 6) & 1))
+                    saw |= has80(st.r[i].x) | has80(st.r[i].y) | has80(st.r[i].z) | has80(st.r[i].w);
+            }
+        };
+        const int8_t* last = src + (size_t)(n_kb - 1) * m.tb;
+        auto at = [&](int kb) { return kb < n_kb ? src + (size_t)kb * m.tb : last; };
+        auto set = [&](auto Sc) -> St& {
+            constexpr int S = decltype(Sc)::value;
+            if constexpr (S == 0) return s0;
+            else if constexpr (S == 1) return s1;
+            else return s2;
+        };
+        auto step = [&](auto Jc, int kb) {
+            constexpr int J = decltype(Jc)::value;
+            St& ld = set(std::integral_constant<int, J % 3>{});
+            St& sv = set(std::integral_constant<int, (J + 1) % 3>{});
+            gload(ld, at(kb + J + 3));
+            block_mfma<NE, RA, A0, A1, RB, B0, B1>((J & 1) ? lds1 : lds0, lrow, kg, ar, ai, br, bi);
+            if (chk) check(sv);
+            sv.store((J & 1) ? lds0 : lds1, ident, ltid);
+            __syncthreads();
+        };
+        gload(s0, src);
+        gload(s1, at(1));
+        gload(s2, at(2));
+        if (chk) check(s0);
+        s0.store(lds0, ident, ltid);
+        __syncthreads();
+        int kb = 0;
+        for (; kb + 6 <= n_kb; kb += 6) static_for<0, 6>([&](auto Jc) { step(Jc, kb); });
+        static_for<0, 5>([&](auto Jc) {  // the last n_kb % 6 blocks
+            if (kb + decltype(Jc)::value < n_kb) step(Jc, kb);
+        });
     } else if constexpr (V == 1) {
         auto ident = [](int b) { return b; };
         using St = Stager<NE, NBLK, NTH>;

@@ -1,5 +1,5 @@
 // kernels_corr.hip -- dispatch of the correlator launch plan to the kernel variants
-// (corr_impl.h; instantiated per pipeline variant in corr_{fold,blk}_pf{1,2,3,4}.hip and, for the
+// (corr_impl.h; instantiated per pipeline variant in corr_{fold,blk}_pf{1,2,3,4,5}.hip and, for the
 // 16x16x64 tile shape, corr_fold_m16.hip).
 #include "kernels.h"
 
@@ -13,10 +13,12 @@ MIXER_DECL(fold, 1)
 MIXER_DECL(fold, 2)
 MIXER_DECL(fold, 3)
 MIXER_DECL(fold, 4)
+MIXER_DECL(fold, 5)
 MIXER_DECL(blk, 1)
 MIXER_DECL(blk, 2)
 MIXER_DECL(blk, 3)
 MIXER_DECL(blk, 4)
+MIXER_DECL(blk, 5)
 #undef MIXER_DECL
 
 hipError_t launch_fold_m16(hipStream_t s, const XcorrPlan& p, const XSrc& src, int32_t* acc,
@@ -44,6 +46,8 @@ hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& p, const XSrc& src, int3
                         : launch_blk_3(s, p, src, acc, wrap, C, n_tb, accumulate);
     case 4: return fold ? launch_fold_4(s, p, src, acc, wrap, C, n_tb, accumulate)
                         : launch_blk_4(s, p, src, acc, wrap, C, n_tb, accumulate);
+    case 5: return fold ? launch_fold_5(s, p, src, acc, wrap, C, n_tb, accumulate)
+                        : launch_blk_5(s, p, src, acc, wrap, C, n_tb, accumulate);
     default: return hipErrorInvalidValue;
     }
 }

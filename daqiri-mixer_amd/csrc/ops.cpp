@@ -404,7 +404,7 @@ int mixer_image(void* stream, const mixer_array_cfg* c, const int32_t* d_acc, fl
 int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
     if (!t || n_elem < 32 || n_elem > 1024 || n_elem % 32) return MIXER_ERR_ARG;
     if (t->slots < 0 || t->slots > 8) return MIXER_ERR_ARG;
-    if (t->prefetch < 0 || t->prefetch > 4) return MIXER_ERR_ARG;
+    if (t->prefetch < 0 || t->prefetch > 5) return MIXER_ERR_ARG;
     if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
     if (t->staging < 0 || t->staging > 2) return MIXER_ERR_ARG;
     if (t->mfma != 0 && t->mfma != 16 && t->mfma != 32) return MIXER_ERR_ARG;

@@ -97,14 +97,14 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
     int st = MIXER_OK;
     // the 32x32x32 shape with each pipeline variant, then (arrays of <= 8 element blocks) the
     // 16x16x64 shape, whose two-block stages have a pipeline of their own
-    const int n_pf = nb <= 8 ? 5 : 4;
+    const int n_pf = nb <= 8 ? 6 : 5;
     for (int staging : {1, 2}) {
         for (int slots : {1, 2, 4, 8}) {
             if (slots > n_tb) continue;
             for (int k = 0; k < n_pf; ++k) {
                 t.slots = slots;
-                t.prefetch = k < 4 ? k + 1 : 0;
-                t.mfma = k < 4 ? 32 : 16;
+                t.prefetch = k < 5 ? k + 1 : 0;
+                t.mfma = k < 5 ? 32 : 16;
                 t.staging = staging;
                 if ((st = mixer_set_tuning(c.n_elem, &t))) break;
                 float ms = 0;

@@ -154,8 +154,9 @@ int mixer_image_noalloc(void* stream, const mixer_array_cfg* cfg, const int32_t*
 /* Launch tuning (the autotuner's search space), per array size class n_elem:
  *   slots     time split of one correlator call into int32 partial sums (1..8)
  *   prefetch  correlator time-loop pipeline: 1 = register staging one 32-sample block ahead,
- *             3 = register staging two blocks ahead, 2 or 4 = direct global->LDS loads, that
- *             many blocks per barrier
+ *             3 = register staging two blocks ahead, 5 = register staging three blocks ahead
+ *             (fold and blocked layouts), 2 or 4 = direct global->LDS loads, that many blocks
+ *             per barrier
  *   ct_chan   corner-turn channels per workgroup (8, 16 or 32)
  *   staging   how contexts feed the correlator: 1 = straight from the heaps (the corner-turn
  *             is fused into the correlator's loads, mixer_xcorr_heaps), 2 = a separate

@@ -350,7 +350,7 @@ def test_every_launch_variant_bit_exact(N, C):
     d_bb = M.DeviceBuffer.from_array(bb)
     try:
         for slots in (1, 2, 8):
-            for pf in (1, 2, 3, 4):
+            for pf in (1, 2, 3, 4, 5):
                 for ct in (8, 16, 32):
                     if ct != 32 and (slots, pf) != (2, 2):
                         continue
@@ -374,7 +374,7 @@ def test_autotune_sets_a_valid_plan():
         best = M.autotune(cfg)
         log(f"autotune 128 x 16 x 1024 -> {best.as_dict()}")
         assert best.slots in (1, 2, 4, 8) and best.ct_chan in (8, 16, 32) and best.staging in (1, 2)
-        assert (best.mfma, best.prefetch) in [(32, 1), (32, 2), (32, 3), (32, 4), (16, 0)]
+        assert (best.mfma, best.prefetch) in [(32, 1), (32, 2), (32, 3), (32, 4), (32, 5), (16, 0)]
         t = M.get_tuning(cfg)
         assert (t.slots, t.ct_chan, t.mfma) == (best.slots, best.ct_chan, best.mfma)
         assert best.mfma == 16 or t.prefetch == best.prefetch
@@ -394,6 +394,8 @@ def test_autotune_sets_a_valid_plan():
     (224, 2, 3, 1), (384, 2, 2, 1), (512, 3, 3, 1), (1024, 1, 2, 1), (640, 1, 4, 2),
     # odd block counts per time slot (7; 3 and 4) through the two-ahead pipelines' last pass
     (256, 6, 7, 1), (512, 2, 7, 1),
+    # the three-ahead pipeline's six-block rounds: whole rounds, and rounds plus 1..5 blocks
+    (1024, 1, 13, 1), (512, 2, 6, 1), (256, 2, 11, 1), (384, 1, 12, 1), (640, 1, 9, 1),
 ])
 def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
     """The correlator staging straight from the heaps gives the CPU correlator's V, for every
@@ -404,8 +406,8 @@ def test_xcorr_heaps_fused_bit_exact(N, C, n_tb, parts):
     per = n_tb // parts
     try:
         for slots in (1, 2, 8):
-            for pf in (1, 2, 3, 4):
-                if slots > per or (slots == 8 and pf not in (1, 3)):
+            for pf in (1, 2, 3, 4, 5):
+                if slots > per or (slots == 8 and pf not in (1, 3, 5)):
                     continue
                 M.set_tuning(N, slots, pf)
                 cfg = cfg_for(N, C, 32 * per)
@@ -477,7 +479,7 @@ def test_mfma16_shape_counts_minus128_and_runs_the_engine():
 
 
 @pytest.mark.parametrize("N,C,pf", [(256, 4, 1), (256, 4, 3), (256, 4, 4), (96, 3, 2), (160, 2, 3), (512, 2, 1),
-                                    (512, 2, 2), (512, 2, 3)])
+                                    (512, 2, 2), (512, 2, 3), (256, 4, 5), (512, 2, 5)])
 def test_xcorr_heaps_counts_imaginary_minus128(N, C, pf):
     try:
         M.set_tuning(N, prefetch=pf)
