@@ -92,6 +92,7 @@ def algorithmic(cfg):
         "xcorr_ops": 8 * N * (N + 1) // 2 * K * C,                          # Hermitian complex MACs x 8
         "xcorr_exec_ops": nb * (nb + 1) // 2 * 1024 * 4 * 2 * K * C,        # executed MFMA ops
         "integrate_bytes": 2 * C * N * (N + 1) // 2 * 4 + 2 * C * N * N * 4 + 2 * grid,
+        "v_grid_bytes": 2 * C * N * N * 4 + 2 * grid,                      # written V + grid RMW
         "image_bytes": 2 * grid + C * cfg.ng * cfg.ng * 4 + 2 * C * 4,      # read + clear grid, write image, vmax/vmin
     }
 
@@ -295,6 +296,9 @@ def main():
     ap.add_argument("--staging", type=int, default=0, choices=(0, 1, 2),
                     help="1: the correlator stages straight from the heaps (automatic choice); "
                          "2: a separate corner-turn pass into tiles feeds it (to measure the corner-turn)")
+    ap.add_argument("--integrate", type=int, default=0, choices=(0, 1, 2),
+                    help="blocked arrays (384-1024 elements): 2 writes V and the grid from the "
+                         "correlator's epilogue (automatic choice), 1 runs the separate integrate kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} not supported)",
@@ -304,8 +308,8 @@ def main():
     M.check(M.lib().mixer_set_device(0))
 
     cfg = make_cfg(args.config)
-    if args.staging:
-        M.set_tuning(cfg.n_elem, staging=args.staging)
+    if args.staging or args.integrate:
+        M.set_tuning(cfg.n_elem, staging=args.staging, integrate=args.integrate)
     alg = algorithmic(cfg)
     N, C, K = cfg.n_elem, cfg.n_chan, cfg.k_integ
     n_tb = K // 32
@@ -379,11 +383,21 @@ def main():
         kern["corner_turn"].update(bound="hbm", achieved=alg["corner_turn_bytes"] / (kern["corner_turn"]["avg_ms"] * 1e-3) / 1e9,
                                    unit="GB/s", algorithmic_per_launch=alg["corner_turn_bytes"])
     xt = kern["xcorr"]["avg_ms"] * 1e-3
-    kern["xcorr"].update(bound="hbm", achieved=alg["payload_bytes"] / xt / 1e9, unit="GB/s",
-                         algorithmic_per_launch=alg["payload_bytes"],
+    # blocked arrays whose integration is one correlator launch write V and the image grid from the
+    # correlator's epilogue: no integrate launch, and those bytes are the correlator's
+    fused_integ = "integrate" not in kern
+    xbytes = alg["payload_bytes"] + (alg["v_grid_bytes"] if fused_integ else 0)
+    # the bound by arithmetic intensity: int8 ops per input byte against the ridge 5.0 POP/s / 8 TB/s
+    mfma_bound = alg["xcorr_ops"] / alg["payload_bytes"] > INT8_PEAK_TOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    kern["xcorr"].update(bound="mfma" if mfma_bound else "hbm",
+                         achieved=alg["xcorr_ops"] / xt / 1e12 if mfma_bound else xbytes / xt / 1e9,
+                         unit="TOP/s" if mfma_bound else "GB/s", algorithmic_per_launch=xbytes,
                          tops_algorithmic=alg["xcorr_ops"] / xt / 1e12, tops_executed=alg["xcorr_exec_ops"] / xt / 1e12)
-    kern["integrate"].update(bound="hbm", achieved=alg["integrate_bytes"] / (kern["integrate"]["avg_ms"] * 1e-3) / 1e9,
-                             unit="GB/s", algorithmic_per_launch=alg["integrate_bytes"])
+    if fused_integ:
+        kern["xcorr"]["fused_integrate"] = True
+    else:
+        kern["integrate"].update(bound="hbm", achieved=alg["integrate_bytes"] / (kern["integrate"]["avg_ms"] * 1e-3) / 1e9,
+                                 unit="GB/s", algorithmic_per_launch=alg["integrate_bytes"])
     kern["image"].update(bound="hbm", achieved=alg["image_bytes"] / (kern["image"]["avg_ms"] * 1e-3) / 1e9,
                          unit="GB/s", algorithmic_per_launch=alg["image_bytes"])
     for k in kern.values():
@@ -495,7 +509,7 @@ def main():
         cpu = cpu_baseline(runs) if any(r[0] == "cfg2" for r in runs) else None
 
     d = kern[dom]
-    peak = HBM_PEAK_GBS
+    peak = INT8_PEAK_TOPS if d["bound"] == "mfma" else HBM_PEAK_GBS
     # HBM bytes per correlator launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this same command (scaled by a calibration copy); only valid for the launch shape profiled
     traffic = None
@@ -518,7 +532,7 @@ def main():
         "x_400gbe_equivalent": round(value / GBE400_PAYLOAD_GBS, 2),
         "mfma_pct_peak": round(mfma_pct, 2),
         "roofline": {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"], "peak": peak,
-                     "unit": "GB/s", "frac": round(d["achieved"] / peak, 4), "traffic": traffic},
+                     "unit": d["unit"], "frac": round(d["achieved"] / peak, 4), "traffic": traffic},
         "kernels": kern,
         "latency": latency,
         "plain_gpu": plain,

@@ -289,6 +289,24 @@ __device__ __forceinline__ void corr_wave(const int8_t* __restrict__ src, const 
     }
 }
 
+// Where a wave puts a finished 32x32 tile (I, J) of its channel (I >= J, element blocks): the
+// default writes the tile-packed int32 accumulator (tile t = I(I+1)/2 + J, 2048 ints; store or
+// add); the fused integrate epilogue (corr_epi.h) writes V and grids it instead.
+struct AccOut {
+    int32_t* acc;
+    int accumulate;
+    __device__ __forceinline__ void operator()(const v16i& vr, const v16i& vi, int I, int J, int lrow,
+                                               int kg) const {
+        int32_t* t = acc + ((size_t)I * (I + 1) / 2 + J) * 2048;
+        store_tile(vr, t, lrow, kg, accumulate);
+        store_tile(vi, t + 1024, lrow, kg, accumulate);
+    }
+    // workgroup-wide steps around a wave program's tile outputs (every wave of the group calls
+    // both, after its time loop): nothing to do for the accumulator
+    __device__ __forceinline__ void begin() const {}
+    __device__ __forceinline__ void end() const {}
+};
+
 // FOLD wave program: tiles (RA, [A0,A1)) u (RB, [B0,B1)) of the lower triangle of NB blocks
 // starting at element block d0 (0 for a whole array; the diagonal triangles of the BLOCKED
 // decomposition use d0 = 8k); acc points at the channel's tile-packed accumulator (tile
@@ -387,11 +405,10 @@ __device__ __forceinline__ void fold_prog16(const int8_t* __restrict__ src, cons
     }
 }
 
-template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
+template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1, typename OUT>
 __device__ __forceinline__ void fold_prog32(const int8_t* __restrict__ src, const XSrc& m,
-                                            int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
-                                            int n_kb, int accumulate, int ltid, bool chk,
-                                            unsigned& saw, int d0) {
+                                            const OUT& out, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                            int ltid, bool chk, unsigned& saw, int d0) {
     constexpr int NA = A1 - A0;
     constexpr int NBT = RB >= 0 ? B1 - B0 : 1;
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
@@ -404,33 +421,28 @@ __device__ __forceinline__ void fold_prog32(const int8_t* __restrict__ src, cons
                                                            [=](int b) { return d0 + b; }, ar, ai,
                                                            br, bi, ltid, chk, saw);
     const int IA = d0 + RA, IB = d0 + RB;
+    out.begin();
 #pragma unroll
-    for (int j = 0; j < NA; ++j) {
-        int32_t* t = acc + ((size_t)IA * (IA + 1) / 2 + d0 + A0 + j) * 2048;
-        store_tile(ar[j], t, lrow, kg, accumulate);
-        store_tile(ai[j], t + 1024, lrow, kg, accumulate);
-    }
+    for (int j = 0; j < NA; ++j) out(ar[j], ai[j], IA, d0 + A0 + j, lrow, kg);
     if constexpr (RB >= 0) {
 #pragma unroll
-        for (int j = 0; j < NBT; ++j) {
-            int32_t* t = acc + ((size_t)IB * (IB + 1) / 2 + d0 + B0 + j) * 2048;
-            store_tile(br[j], t, lrow, kg, accumulate);
-            store_tile(bi[j], t + 1024, lrow, kg, accumulate);
-        }
+        for (int j = 0; j < NBT; ++j) out(br[j], bi[j], IB, d0 + B0 + j, lrow, kg);
     }
+    out.end();
 }
 
-template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1>
+template <int V, int NB, int NTH, int RA, int A0, int A1, int RB, int B0, int B1, typename OUT>
 __device__ __forceinline__ void fold_prog(const int8_t* __restrict__ src, const XSrc& m,
-                                          int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
-                                          int n_kb, int accumulate, int ltid, bool chk,
-                                          unsigned& saw, int d0) {
-    if constexpr (V == 6)
-        fold_prog16<NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid,
-                                                     chk, saw, d0);
-    else
-        fold_prog32<V, NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, acc, lds0, lds1, n_kb, accumulate,
-                                                        ltid, chk, saw, d0);
+                                          const OUT& out, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                          int ltid, bool chk, unsigned& saw, int d0) {
+    if constexpr (V == 6) {
+        static_assert(std::is_same<OUT, AccOut>::value, "the 16x16x64 programs store accumulators");
+        fold_prog16<NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, out.acc, lds0, lds1, n_kb,
+                                                     out.accumulate, ltid, chk, saw, d0);
+    } else {
+        fold_prog32<V, NB, NTH, RA, A0, A1, RB, B0, B1>(src, m, out, lds0, lds1, n_kb, ltid, chk,
+                                                        saw, d0);
+    }
 }
 
 // Wave layout of the FOLD correlator. Row pair p = (p, NB-1-p) holds T_p = NB+1 tiles (the
@@ -447,31 +459,31 @@ struct FoldLayout {
     static constexpr int NTH = 64 * WAVES;
 };
 
-template <int V, int NB, int W, int SPLIT_FROM = 6>
-__device__ __forceinline__ void fold_run(const int8_t* src, const XSrc& m, int32_t* acc,
-                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                         int ltid, bool chk, unsigned& saw, int d0) {
+template <int V, int NB, int W, int SPLIT_FROM = 6, typename OUT>
+__device__ __forceinline__ void fold_run(const int8_t* src, const XSrc& m, const OUT& out,
+                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int ltid, bool chk,
+                                         unsigned& saw, int d0) {
     using Lay = FoldLayout<NB, SPLIT_FROM>;
     constexpr int NTH = Lay::NTH;
     if constexpr (!Lay::SPLIT) {
         constexpr int R1 = W, R2 = NB - 1 - W;
         if constexpr (R1 == R2)
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
         else
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, 0, R2 + 1>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
     } else {
         constexpr int p = W % Lay::P;
         constexpr int R1 = p, R2 = NB - 1 - p;
         constexpr int T = (R1 == R2) ? R1 + 1 : R1 + R2 + 2;
         constexpr int S = (T + 1) / 2;
         if constexpr (W < Lay::P) {
-            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R2, 0, S, -1, 0, 0>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
         } else if constexpr (R1 == R2) {
-            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R2, S, R2 + 1, -1, 0, 0>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
         } else if constexpr (S > R2) {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, -1, 0, 0>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
         } else {
-            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk, saw, d0);
+            fold_prog<V, NB, NTH, R1, 0, R1 + 1, R2, S, R2 + 1>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
         }
     }
 }
@@ -484,16 +496,15 @@ struct FoldLds {
 
 // One channel (or time slot of it) of the FOLD correlator on the FoldLayout<NB>::NTH threads
 // with local ids ltid in [0, NTH) (the group's wave w = ltid / 64 runs wave program w).
-template <int NB, int V, int SPLIT_FROM = 6>
-__device__ __forceinline__ void fold_item(const int8_t* src, const XSrc& m, int32_t* acc,
-                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                          int ltid, bool chk, unsigned& saw, int d0 = 0) {
+template <int NB, int V, int SPLIT_FROM = 6, typename OUT>
+__device__ __forceinline__ void fold_item(const int8_t* src, const XSrc& m, const OUT& out,
+                                          uint8_t* lds0, uint8_t* lds1, int n_kb, int ltid, bool chk,
+                                          unsigned& saw, int d0 = 0) {
     const int w = ltid >> 6;
     static_for<0, FoldLayout<NB, SPLIT_FROM>::WAVES>([&](auto Wc) {
         constexpr int W = decltype(Wc)::value;
         if (w == W)
-            fold_run<V, NB, W, SPLIT_FROM>(src, m, acc, lds0, lds1, n_kb, accumulate, ltid, chk,
-                                           saw, d0);
+            fold_run<V, NB, W, SPLIT_FROM>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, d0);
     });
 }
 
@@ -532,7 +543,8 @@ void k_xcorr_fold(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
     int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
     const int8_t* src = xs.base + (size_t)c * xs.chan + (size_t)kb0 * xs.tb;
     unsigned saw = 0;
-    fold_item<NB, V>(src, xs, a, lds0, lds1, kb1 - kb0, accumulate, threadIdx.x, wrap != nullptr, saw);
+    fold_item<NB, V>(src, xs, AccOut{a, accumulate}, lds0, lds1, kb1 - kb0, threadIdx.x,
+                     wrap != nullptr, saw);
     count_wrap(wrap, saw);
 }
 
@@ -547,11 +559,10 @@ void k_xcorr_fold(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wra
 // (dispatch is round-robin over the 8 XCDs) so the channel's slab is read from HBM about once
 // per XCD L2.
 // ------------------------------------------------------------------------------------------
-template <int V, int PAT, int RS, int CS, typename GB>
+template <int V, int PAT, int RS, int CS, typename GB, typename OUT>
 __device__ __forceinline__ void blk_prog(const int8_t* __restrict__ src, const XSrc& m,
-                                         int32_t* __restrict__ acc, uint8_t* lds0, uint8_t* lds1,
-                                         int n_kb, GB gblk, int Ib, int Jb, int accumulate,
-                                         int ltid, bool chk, unsigned& saw) {
+                                         const OUT& out, uint8_t* lds0, uint8_t* lds1, int n_kb,
+                                         GB gblk, int Ib, int Jb, int ltid, bool chk, unsigned& saw) {
     // PAT 0 FULL: rows (RS, RS+1) x columns (CS, CS+1); 1 DIAG: (0,0), (1,0), (1,1); 2 EMPTY
     const int lane = threadIdx.x & 63, lrow = lane & 31, kg = lane >> 5;
     v16i ar[2], ai[2], br[2], bi[2];
@@ -563,18 +574,17 @@ __device__ __forceinline__ void blk_prog(const int8_t* __restrict__ src, const X
         corr_wave<V, 12 * 32, 12, 512, RS, CS, CS + 1, RS + 1, CS, CS + 2>(src, m, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid, chk, saw);
     else
         corr_wave<V, 12 * 32, 12, 512, RS, CS, CS, -1, 0, 0>(src, m, lds0, lds1, n_kb, gblk, ar, ai, br, bi, ltid, chk, saw);
+    out.begin();
     if constexpr (PAT != 2) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 if (PAT == 1 && i == 0 && j == 1) continue;
-                const int I = Ib + i, J = Jb + j;
-                int32_t* t = acc + (size_t)(I * (I + 1) / 2 + J) * 2048;
-                store_tile(i == 0 ? ar[j] : br[j], t, lrow, kg, accumulate);
-                store_tile(i == 0 ? ai[j] : bi[j], t + 1024, lrow, kg, accumulate);
+                out(i == 0 ? ar[j] : br[j], i == 0 ? ai[j] : bi[j], Ib + i, Jb + j, lrow, kg);
             }
     }
+    out.end();
 }
 
 template <int V>
@@ -587,14 +597,13 @@ struct BlkLds {
 // (36 tiles, no idle wave); J0 >= 0 is the 4x8-tile super-block at rows I0.., columns J0.. .
 // nb (the array's element blocks) only clamps the staged column blocks, so the batched kernel
 // runs the same wave programs for every array size.
-template <int V>
-__device__ __forceinline__ void blk_item_n(const int8_t* src, const XSrc& m, int32_t* a,
-                                           uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                           int I0, int J0, int nb, int ltid, bool chk,
-                                           unsigned& saw) {
+template <int V, typename OUT>
+__device__ __forceinline__ void blk_item_n(const int8_t* src, const XSrc& m, const OUT& out,
+                                           uint8_t* lds0, uint8_t* lds1, int n_kb, int I0, int J0,
+                                           int nb, int ltid, bool chk, unsigned& saw) {
     static_assert(FoldLds<8, V>::BYTES <= BlkLds<V>::BYTES, "fold stage must fit the blocked LDS");
     if (J0 < 0) {
-        fold_item<8, V>(src, m, a, lds0, lds1, n_kb, accumulate, ltid, chk, saw, I0);
+        fold_item<8, V>(src, m, out, lds0, lds1, n_kb, ltid, chk, saw, I0);
         return;
     }
     auto gblk = [=](int s) {
@@ -608,20 +617,20 @@ __device__ __forceinline__ void blk_item_n(const int8_t* src, const XSrc& m, int
         if (w == W) {
             const int Ib = I0 + RS, Jb = J0 + CS - 4;
             if (Jb + 1 < Ib)
-                blk_prog<V, 0, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
+                blk_prog<V, 0, RS, CS>(src, m, out, lds0, lds1, n_kb, gblk, Ib, Jb, ltid, chk, saw);
             else if (Jb == Ib)
-                blk_prog<V, 1, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
+                blk_prog<V, 1, RS, CS>(src, m, out, lds0, lds1, n_kb, gblk, Ib, Jb, ltid, chk, saw);
             else
-                blk_prog<V, 2, RS, CS>(src, m, a, lds0, lds1, n_kb, gblk, Ib, Jb, accumulate, ltid, chk, saw);
+                blk_prog<V, 2, RS, CS>(src, m, out, lds0, lds1, n_kb, gblk, Ib, Jb, ltid, chk, saw);
         }
     });
 }
 
-template <int NB, int V>
-__device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, int32_t* a,
-                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int accumulate,
-                                         int I0, int J0, int ltid, bool chk, unsigned& saw) {
-    blk_item_n<V>(src, m, a, lds0, lds1, n_kb, accumulate, I0, J0, NB, ltid, chk, saw);
+template <int NB, int V, typename OUT>
+__device__ __forceinline__ void blk_item(const int8_t* src, const XSrc& m, const OUT& out,
+                                         uint8_t* lds0, uint8_t* lds1, int n_kb, int I0, int J0,
+                                         int ltid, bool chk, unsigned& saw) {
+    blk_item_n<V>(src, m, out, lds0, lds1, n_kb, I0, J0, NB, ltid, chk, saw);
 }
 
 template <int NB, int V>
@@ -645,7 +654,7 @@ void k_xcorr_blk(XSrc xs, int32_t* __restrict__ acc, uint32_t* __restrict__ wrap
     const int8_t* src = xs.base + (size_t)c * xs.chan + (size_t)kb0 * xs.tb;
     int32_t* a = acc + ((size_t)slot * C + c) * NT * 2048;
     unsigned saw = 0;
-    blk_item<NB, V>(src, xs, a, lds0, lds1, kb1 - kb0, accumulate, sb.x, sb.y, threadIdx.x,
+    blk_item<NB, V>(src, xs, AccOut{a, accumulate}, lds0, lds1, kb1 - kb0, sb.x, sb.y, threadIdx.x,
                     wrap != nullptr, saw);
     count_wrap(wrap, saw);
 }

@@ -58,6 +58,7 @@ struct mixer_ctx {
     int n_slots = 1;
     int n_tb_upd = 0;       // time blocks per V-update
     int upd_per_integ = 0;  // V-updates per integration
+    bool fuse_integ = false;  // one correlator launch per integration writes V and the grid
     size_t payload = 0;     // bytes per packet payload
     size_t tb_bytes = 0;    // bytes per time block (n_elem payloads)
     size_t v_bytes = 0;     // f32 cube bytes
@@ -184,20 +185,33 @@ void poll_pending(mixer_ctx* x) {
     }
 }
 
-// Integrate + image of the completed integration (s_comp).
-int finish_integration(mixer_ctx* x) {
+// Integrate + image of the completed integration (s_comp). With fused_src (one correlator launch
+// covers the integration, fuse_integ) the correlator itself writes V and the grid from its
+// epilogue, after the same waits for the reader.
+int finish_integration(mixer_ctx* x, const XSrc* fused_src) {
     std::lock_guard<std::mutex> lk(x->mtx);
     const uint64_t idx = x->n_integ.load();
     const int b = (int)(idx % 2);
     MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pub_done[b], 0));
     if (x->d_vcum) MIXER_HIP(hipStreamWaitEvent(x->s_comp, x->pub_done_cum, 0));
     hipEvent_t t0 = nullptr;
-    MIXER_TRY(time_begin(x, &t0, 2));
     const bool image = !(x->cfg.flags & MIXER_CFG_NO_IMAGE);
-    MIXER_HIP(launch_integrate(x->s_comp, x->plan, x->d_acc, x->d_v[b], x->d_vcum,
-                               image ? x->d_G : nullptr, x->cfg.n_elem, x->cfg.n_chan,
-                               x->cfg.grid_x, x->cfg.ng));
-    MIXER_TRY(time_end(x, 2, t0));
+    if (fused_src) {
+        MIXER_TRY(time_begin(x, &t0, 1));
+        const VOutArgs vo{x->d_v[b], x->d_vcum, image ? x->d_G : nullptr, x->cfg.n_elem,
+                          x->cfg.grid_x, x->cfg.ng};
+        MIXER_HIP(launch_xcorr_v(x->s_comp, x->plan, *fused_src, vo, x->d_wrap, x->cfg.n_chan,
+                                 x->n_tb_upd));
+        MIXER_TRY(time_end(x, 1, t0));
+        x->st.vupdates++;
+        x->st.time_blocks += x->n_tb_upd;
+    } else {
+        MIXER_TRY(time_begin(x, &t0, 2));
+        MIXER_HIP(launch_integrate(x->s_comp, x->plan, x->d_acc, x->d_v[b], x->d_vcum,
+                                   image ? x->d_G : nullptr, x->cfg.n_elem, x->cfg.n_chan,
+                                   x->cfg.grid_x, x->cfg.ng));
+        MIXER_TRY(time_end(x, 2, t0));
+    }
     if (image) {
         MIXER_TRY(time_begin(x, &t0, 3));
         float* img = x->d_img[b];
@@ -233,6 +247,10 @@ int run_update(mixer_ctx* x, const uint8_t* src) {
 }
 // Correlate one V-update (from the heaps at src when fused, else from the tiles).
 int run_gemm(mixer_ctx* x, const uint8_t* src) {
+    if (x->fuse_integ) {
+        const XSrc xs = xsrc_heaps(src, x->cfg.n_elem, x->cfg.n_chan);
+        return finish_integration(x, &xs);
+    }
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0, 1));
     const XSrc xs = x->plan.fused ? xsrc_heaps(src, x->cfg.n_elem, x->cfg.n_chan)
@@ -246,7 +264,7 @@ int run_gemm(mixer_ctx* x, const uint8_t* src) {
         x->st.vupdates++;
         x->st.time_blocks += x->n_tb_upd;
     }
-    if (x->upd_in_integ == x->upd_per_integ) MIXER_TRY(finish_integration(x));
+    if (x->upd_in_integ == x->upd_per_integ) MIXER_TRY(finish_integration(x, nullptr));
     return MIXER_OK;
 }
 
@@ -360,6 +378,7 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
     x->n_slots = x->plan.n_slots;
     x->n_tb_upd = cfg->k_update / cfg->t_pkt;
     x->upd_per_integ = cfg->k_integ / cfg->k_update;
+    x->fuse_integ = x->plan.fuse_integ && x->upd_per_integ == 1;
     x->payload = mixer_payload_bytes(cfg);
     x->tb_bytes = x->payload * cfg->n_elem;
     x->v_bytes = (size_t)cfg->n_chan * 2 * cfg->n_elem * cfg->n_elem * sizeof(float);
@@ -384,7 +403,7 @@ int create_ctx(const mixer_array_cfg* cfg, int device, mixer_ctx* x) {
         MIXER_HIP(hipMemset(x->d_vcum, 0, x->v_bytes));  // zeroed once (src/rx_host_corr.cu:188)
     }
     if (!x->plan.fused) MIXER_HIP(hipMalloc(&x->d_tiles, mixer_tiles_bytes(cfg, x->n_tb_upd)));
-    MIXER_HIP(hipMalloc(&x->d_acc, mixer_acc_bytes(cfg)));
+    if (!x->fuse_integ) MIXER_HIP(hipMalloc(&x->d_acc, mixer_acc_bytes(cfg)));
     MIXER_HIP(hipMalloc(&x->d_wrap, sizeof(uint32_t)));
     MIXER_HIP(hipMemset(x->d_wrap, 0, sizeof(uint32_t)));
     const size_t gbytes = (size_t)cfg->n_chan * cfg->ng * cfg->ng * 16;

@@ -31,6 +31,7 @@ struct XcorrPlan {
     const void* d_tasks;  // BLOCKED only: int2 (I0, J0) per task
     int n_quad;           // QUAD items: int2 (d0, -1) triangle or (r0, c0) rectangle
     const void* d_quad;
+    int fuse_integ;       // BLOCKED, one slot, prefetch 3: the integrate may run in the epilogue
     int n_segs;           // integrate work items
     const void* d_segs;   // int4 (d, j0, j1, 0): tiles (J + d, J), j0 <= J < j1
 };
@@ -63,11 +64,24 @@ __host__ __device__ inline XSrc xsrc_heaps(const uint8_t* heaps, int N, int C) {
     return XSrc{(const int8_t*)heaps, 32, (size_t)N * payload, C * 32, (int)payload};
 }
 
+// Outputs of the integrate when it is fused into the BLOCKED correlator's epilogue (one launch and
+// one time slot cover the integration): what k_integrate writes from the accumulator.
+struct VOutArgs {
+    float* v_store;         // V [C][2][N][N] f32, overwritten (may be null)
+    float* v_add;           // V [C][2][N][N] f32, added to (the cumulative cube; may be null)
+    unsigned long long* G;  // image grid [C][ng * ng][2] int64 (may be null: no gridding)
+    int N, gs, ng;
+};
+
 hipError_t launch_corner_turn(hipStream_t s, const uint8_t* heaps, int8_t* tiles, uint32_t* wrap,
                               int N, int C, int n_tb, int c_base, int c_count, int cch);
 // wrap (optional): counts workgroups that staged an imaginary -128
 hipError_t launch_xcorr(hipStream_t s, const XcorrPlan& plan, const XSrc& src, int32_t* acc,
                         uint32_t* wrap, int C, int n_tb, int accumulate);
+// BLOCKED plans of one time slot: correlate the whole integration and write V (and grid it) from
+// the epilogue, without the int32 accumulator (prefetch 3)
+hipError_t launch_xcorr_v(hipStream_t s, const XcorrPlan& plan, const XSrc& src, const VOutArgs& vo,
+                          uint32_t* wrap, int C, int n_tb);
 // integrate (+ grid into G when non-null) the plan's accumulator slots, one workgroup per
 // tile-diagonal segment of the plan and channel
 hipError_t launch_integrate(hipStream_t s, const XcorrPlan& p, const int32_t* acc, float* v_store,

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(512) void k_batch_xcorr_blk(const BatchJobDev* __re
             const int8_t* src = m.base + (size_t)c * m.chan + (size_t)kb0 * m.tb;
             int32_t* a = J.acc + ((size_t)slot * J.n_chan + c) * NT * 2048;
             unsigned saw = 0;
-            blk_item_n<3>(src, m, a, lds0, lds1, kb1 - kb0, 0, sb.x, sb.y, nb, threadIdx.x, false,
+            blk_item_n<3>(src, m, AccOut{a, 0}, lds0, lds1, kb1 - kb0, sb.x, sb.y, nb, threadIdx.x, false,
                           saw);
         }
         __syncthreads();  // LDS reuse by the next item

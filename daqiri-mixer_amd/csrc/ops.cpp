@@ -32,7 +32,7 @@ std::map<int, mixer_tuning> g_tuning;  // n_elem -> tuning (0 fields = automatic
 mixer_tuning tuning_for(int n_elem) {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     auto it = g_tuning.find(n_elem);
-    if (it == g_tuning.end()) return mixer_tuning{0, 0, 0, 0, 0};
+    if (it == g_tuning.end()) return mixer_tuning{0, 0, 0, 0, 0, 0};
     return it->second;
 }
 
@@ -170,6 +170,11 @@ int get_plan(const mixer_array_cfg& c, XcorrPlan* p) {
         p->d_segs = it->second;
     }
     p->kind = (XcorrPlan::Kind)plan_kind(c);
+    // BLOCKED with one time slot and register staging two blocks ahead: the integrate can run in
+    // the correlator's epilogue when one launch covers an integration (1024 x 256 x 4096: no
+    // 1.1 GB accumulator store and read-back, no integrate launch)
+    p->fuse_integ = p->kind == XcorrPlan::BLOCKED && p->n_slots == 1 && p->prefetch == 3 &&
+                    p->fused && t.integrate != 1;
     p->n_quad = 0;
     p->d_quad = nullptr;
     if (nb % 4 == 0) {
@@ -408,6 +413,7 @@ int mixer_set_tuning(int n_elem, const mixer_tuning* t) {
     if (t->ct_chan != 0 && t->ct_chan != 8 && t->ct_chan != 16 && t->ct_chan != 32) return MIXER_ERR_ARG;
     if (t->staging < 0 || t->staging > 2) return MIXER_ERR_ARG;
     if (t->mfma != 0 && t->mfma != 16 && t->mfma != 32) return MIXER_ERR_ARG;
+    if (t->integrate < 0 || t->integrate > 2) return MIXER_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     g_tuning[n_elem] = *t;
     return MIXER_OK;
@@ -424,6 +430,7 @@ int mixer_get_tuning(const mixer_array_cfg* c, mixer_tuning* t) {
     t->ct_chan = p.ct_ch;
     t->staging = p.fused ? 1 : 2;
     t->mfma = p.mfma;
+    t->integrate = p.fuse_integ ? 2 : 1;
     return MIXER_OK;
 }
 

@@ -42,10 +42,15 @@ int run_once(const mixer_array_cfg& c, Bufs& b, int n_tb, bool ct_only) {
                                      c.n_chan, p.ct_ch));
     if (ct_only) return MIXER_OK;
     const XSrc xs = p.fused ? xsrc_heaps(b.heaps, c.n_elem, c.n_chan) : xsrc_tiles(b.tiles, c.n_elem, n_tb);
-    MIXER_HIP(launch_xcorr(b.s, p, xs, b.acc, nullptr, c.n_chan, n_tb, 0));
     const float2* tw = nullptr;
     MIXER_TRY(get_twiddles(c.ng, &tw));
-    MIXER_HIP(launch_integrate(b.s, p, b.acc, b.v, nullptr, b.G, c.n_elem, c.n_chan, c.grid_x, c.ng));
+    if (p.fuse_integ) {
+        const VOutArgs vo{b.v, nullptr, b.G, c.n_elem, c.grid_x, c.ng};
+        MIXER_HIP(launch_xcorr_v(b.s, p, xs, vo, nullptr, c.n_chan, n_tb));
+    } else {
+        MIXER_HIP(launch_xcorr(b.s, p, xs, b.acc, nullptr, c.n_chan, n_tb, 0));
+        MIXER_HIP(launch_integrate(b.s, p, b.acc, b.v, nullptr, b.G, c.n_elem, c.n_chan, c.grid_x, c.ng));
+    }
     const size_t nimg = (size_t)c.n_chan * c.ng * c.ng;
     MIXER_HIP(launch_image(b.s, b.G, tw, b.img, b.img + nimg, b.img + nimg + c.n_chan, c.n_chan, c.ng));
     return MIXER_OK;
@@ -92,7 +97,7 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
 
     const mixer_tuning saved = tuning_for(c.n_elem);
     const bool had = has_tuning(c.n_elem);
-    mixer_tuning t{0, 0, 32, 0, 32}, win{0, 0, 32, 0, 32};
+    mixer_tuning t{0, 0, 32, 0, 32, 0}, win{0, 0, 32, 0, 32, 0};
     float best_ms = 1e30f;
     int st = MIXER_OK;
     // the 32x32x32 shape with each pipeline variant, then (arrays of <= 8 element blocks) the
@@ -126,8 +131,19 @@ extern "C" int mixer_autotune(const mixer_array_cfg* cfg, int device, mixer_tuni
             if (ms < best_ms) { best_ms = ms; win.ct_chan = ct; }
         }
     }
+    if (!st) {  // a blocked winner with one slot: integrate fused into the epilogue or separate
+        XcorrPlan wp;
+        if (!(st = mixer_set_tuning(c.n_elem, &win)) && !(st = get_plan(c, &wp)) && wp.fuse_integ) {
+            float fused_ms = 0, sep_ms = 0;
+            t = win;
+            t.integrate = 1;
+            if (!(st = time_it(c, b, n_tb, false, &fused_ms)) && !(st = mixer_set_tuning(c.n_elem, &t)) &&
+                !(st = time_it(c, b, n_tb, false, &sep_ms)))
+                win.integrate = sep_ms < fused_ms ? 1 : 2;
+        }
+    }
     if (st) {  // leave the table as it was
-        const mixer_tuning automatic{0, 0, 0, 0, 0};
+        const mixer_tuning automatic{0, 0, 0, 0, 0, 0};
         (void)mixer_set_tuning(c.n_elem, had ? &saved : &automatic);
         return st;
     }

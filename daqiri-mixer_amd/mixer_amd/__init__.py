@@ -51,11 +51,11 @@ class Stats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("slots", C.c_int32), ("prefetch", C.c_int32), ("ct_chan", C.c_int32), ("staging", C.c_int32),
-                ("mfma", C.c_int32)]
+                ("mfma", C.c_int32), ("integrate", C.c_int32)]
 
     def as_dict(self):
         return {"slots": self.slots, "prefetch": self.prefetch, "ct_chan": self.ct_chan,
-                "staging": self.staging, "mfma": self.mfma}
+                "staging": self.staging, "mfma": self.mfma, "integrate": self.integrate}
 
 
 class BatchTuning(C.Structure):
@@ -378,9 +378,11 @@ def xcorr_plain(cfg, d_heaps, n_tb, d_tiles, d_vre, d_vim, stream=None):
           "mixer_xcorr_plain")
 
 
-def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0, staging=0, mfma=0):
-    """mfma: 32 = v_mfma_i32_32x32x32_i8 tiles, 16 = v_mfma_i32_16x16x64_i8 (<= 256 elements), 0 = auto."""
-    t = Tuning(slots, prefetch, ct_chan, staging, mfma)
+def set_tuning(n_elem: int, slots=0, prefetch=0, ct_chan=0, staging=0, mfma=0, integrate=0):
+    """mfma: 32 = v_mfma_i32_32x32x32_i8 tiles, 16 = v_mfma_i32_16x16x64_i8 (<= 256 elements), 0 = auto.
+    integrate: 2 = fused into the blocked correlator's epilogue where one launch covers an
+    integration, 1 = the separate integrate kernel, 0 = auto (2)."""
+    t = Tuning(slots, prefetch, ct_chan, staging, mfma, integrate)
     check(lib().mixer_set_tuning(n_elem, C.byref(t)), "mixer_set_tuning")
 
 

@@ -165,8 +165,13 @@ int mixer_image_noalloc(void* stream, const mixer_array_cfg* cfg, const int32_t*
  *             instruction), 16 = v_mfma_i32_16x16x64_i8 (each 32x32 tile as four 16x16 tiles,
  *             two 32-sample blocks per instruction and per barrier; arrays of <= 256 elements,
  *             on the FOLD layout, prefetch ignored)
- * 0 in a field means automatic: staging 1 with prefetch 3 and the 32x32x32 shape (the measured
- * fastest choice on MI355X for both the fold and the blocked correlator). The table is
+ *   integrate contexts on the BLOCKED layout (384..1024 elements) whose integration is one
+ *             correlator launch of one time slot (k_update = k_integ, prefetch 3, staging 1):
+ *             2 = write V and grid it from the correlator's epilogue (no int32 accumulator round
+ *             trip, no integrate launch), 1 = the separate integrate kernel. mixer_xcorr and
+ *             mixer_xcorr_heaps always fill the accumulator.
+ * 0 in a field means automatic: staging 1 with prefetch 3, the 32x32x32 shape (the measured
+ * fastest choice on MI355X for both the fold and the blocked correlator) and integrate 2. The table is
  * process-wide. slots changes the accumulator
  * layout (mixer_acc_slots, mixer_acc_bytes): set it before allocating accumulators or creating
  * contexts. No reference counterpart (the reference fixes its launch shape at compile time,
@@ -177,6 +182,7 @@ typedef struct mixer_tuning {
     int32_t ct_chan;
     int32_t staging;
     int32_t mfma;
+    int32_t integrate;
 } mixer_tuning;
 int mixer_set_tuning(int n_elem, const mixer_tuning* t);
 int mixer_get_tuning(const mixer_array_cfg* cfg, mixer_tuning* t); /* the plan in effect */

@@ -162,3 +162,47 @@ def test_frame_is_the_image_of_everything_since_the_previous_frame():
         lat, _, _, idx = e.read_image()  # the latest integration alone
         o = O.image_from_v(vs[4][0], vs[4][1], cfg.grid_x, ng)[0]
         assert idx == 5 and np.abs(lat - o).max() / np.abs(o).max() <= 1e-5
+
+
+@pytest.mark.parametrize("N,C,gx,n_tb,flags", [(1024, 16, 32, 8, 0), (512, 64, 32, 6, M.CFG_CUMULATIVE),
+                                               (384, 96, 16, 5, 0), (768, 24, 32, 3, M.CFG_CUMULATIVE)])
+def test_fused_integrate_matches_oracle_and_separate_kernel(N, C, gx, n_tb, flags):
+    """Blocked arrays whose integration is one correlator launch of one time slot write V and the
+    image grid from the correlator's epilogue (tuning integrate 2). Over two integrations the
+    dumps (per integration and, with CFG_CUMULATIVE, cumulative) and the images equal those of the
+    separate integrate kernel (integrate 1) bit for bit, and V equals the oracle's exact sums
+    rounded once to f32 (src/rx_host_corr.cu:86-95)."""
+    ng = 64
+    cfg = cfg_for(N, C, 32 * n_tb, gx=gx, ng=ng, flags=flags)
+    pitch = int(cfg.pkt_pitch)
+    sky = M.default_sky(sigma=0.5, planet_l=0.3, planet_m=-0.2)
+    pk = M.synth_packets(cfg, sky, seed=31 + N, seq0=0, npkts=2 * n_tb * N, pitch=pitch)
+    out = {}
+    try:
+        for integ in (2, 1):
+            M.set_tuning(N, slots=1, prefetch=3, integrate=integ)
+            t = M.get_tuning(cfg)
+            assert t.integrate == integ and t.slots == 1, t.as_dict()
+            log(f"fused integrate N={N} C={C} gx={gx} n_tb={n_tb} flags={flags}: integrate={integ}")
+            res = []
+            with M.Engine(cfg) as e:
+                for k in range(2):
+                    e.token_wait(e.submit_heaps(pk[k * n_tb * N:(k + 1) * n_tb * N], pitch, n_tb * N))
+                    e.sync()
+                    v, idx = e.dump()
+                    img, vmax, vmin, iidx = e.read_image()
+                    cum = e.dump(cumulative=True)[0] if flags & M.CFG_CUMULATIVE else None
+                    assert idx == k + 1 and iidx == k + 1
+                    res.append((v, img, vmax, vmin, cum))
+            out[integ] = res
+    finally:
+        M.set_tuning(N)
+    for k in range(2):
+        for a, b in zip(out[2][k], out[1][k]):
+            if a is not None:
+                assert np.array_equal(a, b)
+    bb = heaps_of(pk, N, C)
+    for k in range(2):
+        assert exact(out[2][k][0], bb[k * n_tb:(k + 1) * n_tb], n_tb, N, C)
+    if flags & M.CFG_CUMULATIVE:
+        assert np.array_equal(out[2][1][4], out[2][0][0] + out[2][1][0])
