@@ -140,10 +140,32 @@ struct VOut {
     }
 };
 
+// The epilogue of work item (I0, J0) (J0 < 0: the 8-block diagonal triangle at I0) of channel c on
+// 512 threads, with the staging buffers lds0 and lds1 (each >= 24 KiB) reused after the time loop:
+// waves 0-4 transpose in lds0, waves 5-7 in lds1, then the cells.
+__device__ __forceinline__ VOut make_vout(const VOutArgs& vo, int c, int I0, int J0, uint8_t* lds0,
+                                          uint8_t* lds1) {
+    const int w = threadIdx.x >> 6;
+    const int R = 32 / vo.gs;
+    // baseline rows of the tiles below the diagonal: d = I - J in [dmin, dmax], d >= 1
+    int dmin, dmax;
+    if (J0 < 0) { dmin = 1; dmax = 7; }
+    else { dmin = I0 - J0 - 7 > 1 ? I0 - J0 - 7 : 1; dmax = I0 + 3 - J0; }
+    VOut out;
+    out.a = vo;
+    out.c = c;
+    out.T = (int*)(w < 5 ? lds0 + w * kEpiT * 4 : lds1 + (w - 5) * kEpiT * 4);
+    out.cells = (unsigned long long*)(lds1 + 3 * kEpiT * 4);
+    out.dy0 = R * dmin - (R - 1);
+    out.ncell = (R * (dmax - dmin) + 2 * R - 1) * (2 * vo.gs - 1);
+    out.ltid = threadIdx.x;
+    out.nth = 512;
+    return out;
+}
+
 // k_xcorr_blk with the fused integrate: one workgroup per (super-block, channel) of one time slot
 // covering the whole integration (n_tb blocks). The wave programs take the array's block count nb
-// at run time (as in the batched kernel), so one instance serves every size. LDS after the time
-// loop: waves 0-4 transpose in lds0, waves 5-7 in lds1, then the cells.
+// at run time (as in the batched kernel), so one instance serves every size.
 template <int V>
 __global__ __launch_bounds__(512)
 void k_xcorr_blk_v(XSrc xs, VOutArgs vo, uint32_t* __restrict__ wrap, const int2* __restrict__ sblocks,
@@ -157,21 +179,7 @@ void k_xcorr_blk_v(XSrc xs, VOutArgs vo, uint32_t* __restrict__ wrap, const int2
     if (logical >= total) return;
     const int c = logical / n_sb;
     const int2 sb = sblocks[logical % n_sb];
-    const int w = threadIdx.x >> 6;
-    const int R = 32 / vo.gs;
-    // baseline rows of the tiles below the diagonal: d = I - J in [dmin, dmax], d >= 1
-    int dmin, dmax;
-    if (sb.y < 0) { dmin = 1; dmax = 7; }
-    else { dmin = sb.x - sb.y - 7 > 1 ? sb.x - sb.y - 7 : 1; dmax = sb.x + 3 - sb.y; }
-    VOut out;
-    out.a = vo;
-    out.c = c;
-    out.T = (int*)(w < 5 ? lds0 + w * kEpiT * 4 : lds1 + (w - 5) * kEpiT * 4);
-    out.cells = (unsigned long long*)(lds1 + 3 * kEpiT * 4);
-    out.dy0 = R * dmin - (R - 1);
-    out.ncell = (R * (dmax - dmin) + 2 * R - 1) * (2 * vo.gs - 1);
-    out.ltid = threadIdx.x;
-    out.nth = 512;
+    const VOut out = make_vout(vo, c, sb.x, sb.y, lds0, lds1);
     const int8_t* src = xs.base + (size_t)c * xs.chan;
     unsigned saw = 0;
     blk_item_n<V>(src, xs, out, lds0, lds1, n_tb, sb.x, sb.y, nb, threadIdx.x, wrap != nullptr, saw);
