@@ -9,10 +9,11 @@
 //     the exact int32 sum;
 //   - for I > J the conjugate block (J, I), through a [32][33] LDS transpose of one plane at a
 //     time (coalesced 16-byte row stores);
-//   - the tile's baselines into the exact int64 image grid G: tiles below the diagonal add into
-//     int64 LDS cells of the workgroup's baseline rows, flushed once per workgroup with their
-//     conjugate mirror at (-dy, -dx); diagonal tiles (I == J, which hold both orientations) add
-//     straight into G, unmirrored.
+//   - the tile's baselines into the exact int64 image grid G, which holds one orientation of each
+//     baseline (the image kernel adds the conjugate mirror): tiles below the diagonal add into
+//     int64 LDS cells of the workgroup's baseline rows, flushed once per workgroup; diagonal
+//     tiles (I == J, which hold both orientations) add their lower baselines and
+//     autocorrelations straight into G.
 // The cell layout follows integrate_seg: with R = 32 / gs, sub-block (ri, rj) and diagonal dd of
 // tile (I, J) are the baseline cell dy = R (I - J) + ri - rj, dx = dd.
 #pragma once
@@ -53,13 +54,7 @@ struct VOut {
         for (int q = ltid; q < ncell; q += nth) {
             const unsigned long long re = cells[2 * q], im = cells[2 * q + 1];
             if (!re && !im) continue;
-            const int dy = dy0 + q / W, dx = q % W - (gs - 1);
-            const int y = ((dy % ng) + ng) % ng, x = dx < 0 ? dx + ng : dx;
-            gadd(&Gc[2 * (y * ng + x)], re);
-            gadd(&Gc[2 * (y * ng + x) + 1], im);
-            const int y2 = y ? ng - y : 0, x2 = x ? ng - x : 0;
-            gadd(&Gc[2 * (y2 * ng + x2)], re);
-            gadd(&Gc[2 * (y2 * ng + x2) + 1], (unsigned long long)(-(long long)im));
+            grid_add_lower(Gc, ng, dy0 + q / W, q % W - (gs - 1), re, im, true);
         }
     }
 
@@ -86,6 +81,8 @@ struct VOut {
         for (int sgi = lane; sgi < nseg; sgi += 64) {
             const int sub = sgi / W, dd = sgi % W - (gs - 1);
             const int ri = sub / R, rj = sub % R;
+            // a diagonal tile holds both orientations: its lower baselines and autocorrelations
+            if (d == 0 && (ri < rj || (ri == rj && dd < 0))) continue;
             const int i0 = dd > 0 ? dd : 0, len = gs - (dd > 0 ? dd : -dd);
             // the diagonal from (ri gs + i0, rj gs + i0 - dd), 34 ints per step; eight reads per
             // round are issued back to back (the workgroup is alone on its CU here, so a chain of
@@ -104,7 +101,7 @@ struct VOut {
                 atomicAdd(&cells[2 * q + p], (unsigned long long)sum);
             } else {
                 const int y = dy < 0 ? dy + ng : dy, x = dd < 0 ? dd + ng : dd;
-                gadd(&Gc[2 * (y * ng + x) + p], (unsigned long long)sum);
+                gadd(&Gc[2 * (y * ng + x) + p], (unsigned long long)sum);  // never aliases to q = 0
             }
         }
     }

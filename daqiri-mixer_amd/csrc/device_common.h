@@ -104,6 +104,24 @@ __device__ __forceinline__ void gadd(unsigned long long* p, unsigned long long v
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The image grid G holds one orientation of every baseline: cell q = (dy, dx) sums V_ab over the
+// pairs a > b (row above column in V) with that cell, plus the autocorrelations V_aa at q = 0. The
+// image kernel completes it to F(q) = G(q) + conj(G(-q)) for q != 0 and F(0) = G(0), since V_ba =
+// conj(V_ab) lies at -q (exact in int64, and half the atomics of adding both orientations). A sum of
+// pairs a > b (off_diag) that lands on q = 0 (only if the grid aliases) counts both orientations
+// there at once: 2 Re.
+__device__ __forceinline__ void grid_add_lower(unsigned long long* Gc, int ng, int dy, int dx,
+                                               unsigned long long re, unsigned long long im,
+                                               bool off_diag) {
+    const int y = ((dy % ng) + ng) % ng, x = ((dx % ng) + ng) % ng;
+    if (off_diag && y == 0 && x == 0) {
+        gadd(&Gc[0], re + re);
+        return;
+    }
+    gadd(&Gc[2 * (y * ng + x)], re);
+    gadd(&Gc[2 * (y * ng + x) + 1], im);
+}
+
 // Copies a list of element blocks of one slab global -> registers -> LDS. NBLK blocks, each
 // 2 planes x 32 elements x 32 samples = 128 chunks of 16 B; NT threads with local ids ltid.
 template <int NE, int NBLK, int NT>

@@ -79,13 +79,17 @@ __device__ __forceinline__ void corner_turn_body(const uint8_t* __restrict__ hea
 // ------------------------------------------------------------------------------------------
 constexpr int kSegCells = 128;  // >= max over gs of (2 * 32 / gs - 1) * (2 * gs - 1) = 105
 
-__device__ __forceinline__ void grid_seg_tile(const int (*T)[32][33], unsigned long long* Gs, int gs) {
+__device__ __forceinline__ void grid_seg_tile(const int (*T)[32][33], unsigned long long* Gs, int gs,
+                                              bool diag) {
     const int R = 32 / gs, W = 2 * gs - 1, nseg = R * R * W;
     const int len = gs > 8 ? 8 : gs, nck = gs / len;
     for (int w = threadIdx.x; w < nseg * nck; w += 256) {
         const int sg = w / nck, k = w % nck;
         const int sub = sg / W, dd = sg % W - (gs - 1);
         const int ri = sub / R, rj = sub % R;
+        // a diagonal tile holds both orientations of its baselines: grid the lower ones (row >
+        // column) and the autocorrelations (dd = 0 on the diagonal sub-blocks) only
+        if (diag && (ri < rj || (ri == rj && dd < 0))) continue;
         const int i0 = dd > 0 ? dd : 0, i1 = dd > 0 ? gs : gs + dd;
         const int c0 = i0 + k * len, c1 = c0 + len < i1 ? c0 + len : i1;
         if (c0 >= c1) continue;
@@ -154,7 +158,7 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
                 T[p][i][jq + 3] = cur[p].w;
             }
             __syncthreads();
-            if (G) grid_seg_tile(T, Gs, gs);
+            if (G) grid_seg_tile(T, Gs, gs, d == 0);
             if (write_v && d > 0) {  // block (J, I) = conj(block (I, J))^T
                 put(J * 32 + i, I * 32 + jq, 0,
                     make_float4((float)T[0][jq][i], (float)T[0][jq + 1][i], (float)T[0][jq + 2][i],
@@ -172,14 +176,7 @@ __device__ __forceinline__ void integrate_seg(const int32_t* __restrict__ acc,
             const unsigned long long re = Gs[2 * q], im = Gs[2 * q + 1];
             if (!re && !im) continue;
             const int dy = R * d + q / W - (R - 1), dx = q % W - (gs - 1);
-            const int y = dy < 0 ? dy + ng : dy, x = dx < 0 ? dx + ng : dx;
-            gadd(&Gc[2 * (y * ng + x)], re);
-            gadd(&Gc[2 * (y * ng + x) + 1], im);
-            if (d > 0) {
-                const int y2 = y ? ng - y : 0, x2 = x ? ng - x : 0;
-                gadd(&Gc[2 * (y2 * ng + x2)], re);
-                gadd(&Gc[2 * (y2 * ng + x2) + 1], (unsigned long long)(-(long long)im));
-            }
+            grid_add_lower(Gc, ng, dy, dx, re, im, d > 0);
         }
     }
 }
@@ -257,17 +254,26 @@ __device__ __forceinline__ void image32_body(unsigned long long* __restrict__ G,
     float2 v[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {  // row y = line, column x = r + 8 m
-        const int q = line * ng + r + 8 * m;
+        const int x = r + 8 * m, q = line * ng + x;
+        const int qm = ((ng - line) & (ng - 1)) * ng + ((ng - x) & (ng - 1));  // -q
+        // F(q) = G(q) + conj(G(-q)), F(0) = G(0); the mirror loads are unconditional so that all
+        // eight go out together (-0 = 0 reads the cell itself)
         const unsigned long long gr = gld(&Gc[2 * q]), gi = gld(&Gc[2 * q + 1]);
-        v[m] = make_float2((float)(long long)gr, (float)(long long)gi);
-        gst(&Gc[2 * q], 0ull);
-        gst(&Gc[2 * q + 1], 0ull);
+        const unsigned long long mr = gld(&Gc[2 * qm]), mi = gld(&Gc[2 * qm + 1]);
+        const unsigned long long fr = q ? gr + mr : gr, fi = q ? gi - mi : gi;
+        v[m] = make_float2((float)(long long)fr, (float)(long long)fi);
         if (Gf) frame_add(Gf + (size_t)c * 2 * ng * ng, q, gr, gi);
     }
     fft32_line(v, r, tw);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) T[line * TS + k1 + 4 * br] = v[k1];
-    __syncthreads();
+    __syncthreads();  // every cell and its mirror are read: clear the grid for the next integration
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int q = line * ng + r + 8 * m;
+        gst(&Gc[2 * q], 0ull);
+        gst(&Gc[2 * q + 1], 0ull);
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) v[m] = T[(r + 8 * m) * TS + line];  // column kx = line
     fft32_line(v, r, tw);
@@ -312,13 +318,20 @@ __device__ __forceinline__ void image_body(unsigned long long* __restrict__ G,
     const int tid = threadIdx.x;
     unsigned long long* Gc = G + (size_t)c * 2 * nn;
     for (int q = tid; q < nn; q += 256) {
+        const int y = q / ng, x = q % ng;
+        const int qm = ((ng - y) & (ng - 1)) * ng + ((ng - x) & (ng - 1));  // -q
+        // F(q) = G(q) + conj(G(-q)), F(0) = G(0)
         const unsigned long long gr = gld(&Gc[2 * q]), gi = gld(&Gc[2 * q + 1]);
-        F0[q] = make_float2((float)(long long)gr, (float)(long long)gi);
-        gst(&Gc[2 * q], 0ull);
-        gst(&Gc[2 * q + 1], 0ull);
+        const unsigned long long mr = gld(&Gc[2 * qm]), mi = gld(&Gc[2 * qm + 1]);
+        const unsigned long long fr = q ? gr + mr : gr, fi = q ? gi - mi : gi;
+        F0[q] = make_float2((float)(long long)fr, (float)(long long)fi);
         if (Gf) frame_add(Gf + (size_t)c * 2 * nn, q, gr, gi);
     }
-    __syncthreads();
+    __syncthreads();  // every cell and its mirror are read: clear the grid for the next integration
+    for (int q = tid; q < nn; q += 256) {
+        gst(&Gc[2 * q], 0ull);
+        gst(&Gc[2 * q + 1], 0ull);
+    }
     float2* src = F0;
     float2* dst = F1;
     const int half = ng >> 1;

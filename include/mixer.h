@@ -24,7 +24,9 @@
  * Hermitian and every output is bit-identical to the reference's full-square product. An
  * imaginary sample of -128 is counted (mixer_stats.wrap128): the lower triangle and the
  * diagonal tiles stay bit-identical to the reference even then, but the mirrored upper
- * triangle is conj(lower), which the reference's wrapped negation does not reproduce.
+ * triangle is conj(lower), which the reference's wrapped negation does not reproduce. The image
+ * grids each baseline's lower orientation and its conjugate, so it is the image of the Hermitian
+ * V that conj(lower) defines (inside the value domain: of V itself).
  */
 #ifndef MIXER_H
 #define MIXER_H
