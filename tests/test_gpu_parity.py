@@ -117,6 +117,71 @@ def test_xcorr_extremes_and_single_element():
     assert np.count_nonzero(vre) == C and np.count_nonzero(vim) == 0
 
 
+K_MAX = 65504  # the longest integration a context accepts (longer ones: MIXER_ERR_OVERFLOW)
+LIMIT_CASES = [(64, 3), (256, 4), (1024, 2)]  # fold, QUAD (two time slots), blocked (fused integrate)
+
+
+def max_magnitude_block(N, C, rng):
+    """One 32-sample heap block whose samples are constant in time and as large as int8 allows:
+    real -128 or 127, imaginary -127 or 127 (an imaginary -128 wraps when negated)."""
+    r = np.where(rng.random((N, C)) < 0.5, -128, 127).astype(np.int8)
+    i = np.where(rng.random((N, C)) < 0.5, -127, 127).astype(np.int8)
+    r[0] = -128  # at least one autocorrelation reaches the bound
+    blk = np.empty((N, 64 * C), np.int8)
+    blk[:, :32 * C] = np.repeat(r, 32, axis=1)
+    blk[:, 32 * C:] = np.repeat(i, 32, axis=1)
+    return blk.view(np.uint8)
+
+
+def limit_case(N, C):
+    """Heaps of K_MAX samples (2047 blocks, an odd count) repeating one maximal block, and the
+    exact V: 2047 times the one-block V of the oracle. The largest |V| is the autocorrelation of
+    an element with real -128: 65504 (128^2 + 127^2) = 2 129 732 352, 99.2 % of 2^31 - 1."""
+    n_tb = K_MAX // 32
+    blk = max_magnitude_block(N, C, RNG)
+    r1, i1 = O.v_full_from_heaps(np.ascontiguousarray(blk[None]), 1, N, C, 32)
+    ore, oim = r1 * n_tb, i1 * n_tb
+    assert np.abs(ore).max() == K_MAX * (128 * 128 + 127 * 127) < 2 ** 31
+    bb = np.ascontiguousarray(np.broadcast_to(blk, (n_tb, N, 64 * C)))
+    return n_tb, bb, np.stack([ore, oim], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("N,C", LIMIT_CASES)
+def test_longest_integration_at_the_int32_limit(N, C):
+    """The int32 sums (src/cgemm.cuh:42-45,58-63) at the longest integration, through a context
+    (default tuning) and through the operator entry points (mixer_xcorr_heaps +
+    mixer_integrate): V is one f32 rounding of the exact sum, bit for bit."""
+    n_tb, bb, exact = limit_case(N, C)
+    d_bb = M.DeviceBuffer.from_array(bb)
+    cfg = cfg_for(N, C, K_MAX)
+    log(f"longest integration N={N} C={C} n_tb={n_tb}: context")
+    with M.Engine(cfg) as e:
+        e.submit_device(d_bb, n_tb, token=False)
+        e.sync()
+        v, idx = e.dump()
+    assert idx == 1 and np.array_equal(v, exact), f"context V differs (N={N})"
+    log(f"longest integration N={N} C={C}: mixer_xcorr_heaps + mixer_integrate")
+    d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+    d_v = M.DeviceBuffer(C * 2 * N * N * 4)
+    M.xcorr_heaps(cfg, d_bb, n_tb, d_acc)
+    M.integrate(cfg, d_acc, d_v)
+    assert np.array_equal(d_v.download(np.float32, (C, 2, N, N)), exact), f"operator V differs (N={N})"
+
+
+def test_longest_integration_at_the_int32_limit_batched():
+    """The same three arrays as one batch: the planner splits their few channels into time slots
+    whose int32 partial sums the batch integrate adds exactly."""
+    cases = [limit_case(N, C) for N, C in LIMIT_CASES]
+    cfgs = [cfg_for(N, C, K_MAX) for N, C in LIMIT_CASES]
+    bufs = [M.DeviceBuffer.from_array(bb) for _, bb, _ in cases]
+    log("longest integration: batch of 3 arrays")
+    with M.Batch(cfgs) as b:
+        b.run(bufs)
+        b.sync()
+        for j, (_, _, exact) in enumerate(cases):
+            assert np.array_equal(b.dump(j), exact), f"batch job {j} {LIMIT_CASES[j]} V differs"
+
+
 def test_xcorr_minus128_lower_triangle_matches_reference():
     """With imaginary -128 present the reference's V is not Hermitian; the lower triangle and
     the diagonal tiles stay bit-identical to it."""
