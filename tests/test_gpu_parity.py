@@ -182,6 +182,36 @@ def test_longest_integration_at_the_int32_limit_batched():
             assert np.array_equal(b.dump(j), exact), f"batch job {j} {LIMIT_CASES[j]} V differs"
 
 
+def channel_v(acc, cfg, c):
+    """Full-square (Vre, Vim) of channel c from a tile-packed accumulator [slots][C][tiles][2][32][32]."""
+    nb = cfg.n_elem // 32
+    one = acc.reshape(-1, cfg.n_chan, nb * (nb + 1) // 2 * 2048)[:, c:c + 1]
+    c1 = cfg_for(cfg.n_elem, 1, cfg.k_update, gx=cfg.grid_x, ng=cfg.ng)
+    vre, vim = M.unpack_acc(np.ascontiguousarray(one).reshape(-1), c1)
+    return vre[0], vim[0]
+
+
+@pytest.mark.parametrize("N,C", [(32, 1024), (256, 1021), (512, 1024)])
+def test_most_channels(N, C):
+    """The largest channel count a context accepts (1024: 64 KiB per heap element, up to 32 MiB
+    per time block) and a ragged 1021 (the last quad of a heap line partly empty) on the fold,
+    QUAD and blocked layouts straight from the heaps: channels 0, 1, C/2, C-2 and C-1
+    bit-exact against the oracle's correlator (src/cgemm.cuh:28-69)."""
+    n_tb = 3
+    bb = rand_heaps(n_tb, N, C)
+    cfg = cfg_for(N, C, 32 * n_tb)
+    log(f"most channels N={N} C={C} n_tb={n_tb}")
+    d_bb = M.DeviceBuffer.from_array(bb)
+    d_acc = M.DeviceBuffer(M.acc_bytes(cfg))
+    M.xcorr_heaps(cfg, d_bb, n_tb, d_acc)
+    acc = d_acc.download(np.int32, (M.acc_bytes(cfg) // 4,))
+    for c in (0, 1, C // 2, C - 2, C - 1):
+        ore, oim, of = O.xengine(bb, n_tb, N, C, 32, c, c + 1, nthreads=min(16, os.cpu_count() or 1))
+        assert of == 0
+        vre, vim = channel_v(acc, cfg, c)
+        assert np.array_equal(vre, ore[0]) and np.array_equal(vim, oim[0]), f"channel {c} differs"
+
+
 def test_xcorr_minus128_lower_triangle_matches_reference():
     """With imaginary -128 present the reference's V is not Hermitian; the lower triangle and
     the diagonal tiles stay bit-identical to it."""
