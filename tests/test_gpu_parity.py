@@ -264,8 +264,12 @@ def test_integrate_f32_and_cumulative():
 
 
 # ---------------------------------------------------------------- image
-@pytest.mark.parametrize("N,gx,C,ng,n_tb", [(256, 16, 4, 32, 4), (1024, 32, 2, 64, 2), (64, 8, 3, 32, 2)])
+@pytest.mark.parametrize("N,gx,C,ng,n_tb", [(256, 16, 4, 32, 4), (1024, 32, 2, 64, 2), (64, 8, 3, 32, 2),
+                                             (32, 4, 3, 16, 2), (64, 8, 2, 16, 2), (256, 16, 2, 64, 2),
+                                             (128, 8, 2, 32, 2), (96, 32, 2, 64, 2)])
 def test_image_vs_fp64(N, gx, C, ng, n_tb):
+    """Every grid size a geometry can ask for (16, 32 in registers, 64) and rectangular arrays
+    (8 x 16, 32 x 3 elements) against the FP64 grid + DFT."""
     log(f"image N={N} gx={gx} C={C} ng={ng}")
     cfg = cfg_for(N, C, 32 * n_tb, gx=gx, ng=ng)
     sky = M.default_sky(sigma=0.5, planet_l=0.5, planet_m=0.25)
