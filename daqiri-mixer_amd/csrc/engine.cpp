@@ -127,6 +127,12 @@ struct mixer_ctx {
 
 namespace {
 
+// The imaginary -128 counter the correlator and the corner-turn add to, or none when the context
+// was created with MIXER_CFG_NO_WRAP_CHECK.
+uint32_t* wrap_of(const mixer_ctx* x) {
+    return (x->cfg.flags & MIXER_CFG_NO_WRAP_CHECK) ? nullptr : x->d_wrap;
+}
+
 int take_event(mixer_ctx* x, hipEvent_t* e) {
     if (!x->ev_pool.empty()) {
         *e = x->ev_pool.back();
@@ -200,7 +206,7 @@ int finish_integration(mixer_ctx* x, const XSrc* fused_src) {
         MIXER_TRY(time_begin(x, &t0, 1));
         const VOutArgs vo{x->d_v[b], x->d_vcum, image ? x->d_G : nullptr, x->cfg.n_elem,
                           x->cfg.grid_x, x->cfg.ng};
-        MIXER_HIP(launch_xcorr_v(x->s_comp, x->plan, *fused_src, vo, x->d_wrap, x->cfg.n_chan,
+        MIXER_HIP(launch_xcorr_v(x->s_comp, x->plan, *fused_src, vo, wrap_of(x), x->cfg.n_chan,
                                  x->n_tb_upd));
         MIXER_TRY(time_end(x, 1, t0));
         x->st.vupdates++;
@@ -240,7 +246,7 @@ int run_update(mixer_ctx* x, const uint8_t* src) {
     if (x->plan.fused) return MIXER_OK;
     hipEvent_t t0 = nullptr;
     MIXER_TRY(time_begin(x, &t0, 0));
-    MIXER_HIP(launch_corner_turn(x->s_comp, src, x->d_tiles, x->d_wrap, x->cfg.n_elem,
+    MIXER_HIP(launch_corner_turn(x->s_comp, src, x->d_tiles, wrap_of(x), x->cfg.n_elem,
                                  x->cfg.n_chan, x->n_tb_upd, 0, x->cfg.n_chan, x->plan.ct_ch));
     MIXER_TRY(time_end(x, 0, t0));
     return MIXER_OK;
@@ -255,7 +261,7 @@ int run_gemm(mixer_ctx* x, const uint8_t* src) {
     MIXER_TRY(time_begin(x, &t0, 1));
     const XSrc xs = x->plan.fused ? xsrc_heaps(src, x->cfg.n_elem, x->cfg.n_chan)
                                   : xsrc_tiles(x->d_tiles, x->cfg.n_elem, x->n_tb_upd);
-    MIXER_HIP(launch_xcorr(x->s_comp, x->plan, xs, x->d_acc, x->plan.fused ? x->d_wrap : nullptr,
+    MIXER_HIP(launch_xcorr(x->s_comp, x->plan, xs, x->d_acc, x->plan.fused ? wrap_of(x) : nullptr,
                            x->cfg.n_chan, x->n_tb_upd, x->upd_in_integ > 0 ? 1 : 0));
     MIXER_TRY(time_end(x, 1, t0));
     x->upd_in_integ++;

@@ -263,17 +263,18 @@ int finish_integration(mixer_server* S, SrvArray* a, uint64_t arrival) {
 // submitted so far.
 int launch_run(mixer_server* S, SrvArray* a) {
     const mixer_array_cfg& c = a->cfg;
+    uint32_t* const wrap = (c.flags & MIXER_CFG_NO_WRAP_CHECK) ? nullptr : a->d_wrap;
     if (a->run_tb > 0) {
         if (a->plan.fused) {
             MIXER_HIP(launch_xcorr(a->s, a->plan, xsrc_heaps(a->run_ptr, c.n_elem, c.n_chan),
-                                   a->d_acc, a->d_wrap, c.n_chan, a->run_tb,
+                                   a->d_acc, wrap, c.n_chan, a->run_tb,
                                    a->upd_launched > 0 ? 1 : 0));
             a->upd_launched += a->run_tb / a->n_tb_upd;
             S->launches++;
         } else {  // the tile buffer holds one update: one corner-turn + correlator per update
             for (int k = 0; k < a->run_tb; k += a->n_tb_upd) {
                 const uint8_t* src = a->run_ptr + (size_t)k * a->tb_bytes;
-                MIXER_HIP(launch_corner_turn(a->s, src, a->d_tiles, a->d_wrap, c.n_elem, c.n_chan,
+                MIXER_HIP(launch_corner_turn(a->s, src, a->d_tiles, wrap, c.n_elem, c.n_chan,
                                              a->n_tb_upd, 0, c.n_chan, a->plan.ct_ch));
                 MIXER_HIP(launch_xcorr(a->s, a->plan, xsrc_tiles(a->d_tiles, c.n_elem, a->n_tb_upd),
                                        a->d_acc, nullptr, c.n_chan, a->n_tb_upd,

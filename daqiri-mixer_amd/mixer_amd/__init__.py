@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmixer.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BUDGET, ERR_NODEV, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
-CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP, CFG_AUTOTUNE, CFG_FRAME_DV = 1, 2, 4, 8, 16
+CFG_CUMULATIVE, CFG_NO_IMAGE, CFG_NOFLOP, CFG_AUTOTUNE, CFG_FRAME_DV, CFG_NO_WRAP_CHECK = 1, 2, 4, 8, 16, 32
 H2D, D2H, D2D = 0, 1, 2
 
 

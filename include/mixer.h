@@ -83,6 +83,10 @@ typedef struct mixer_array_cfg {
                                   everything integrated since the previous frame (the reference's
                                   dV monitor, src/rx_host_corr.cu:218-222), and
                                   mixer_publish_image publishes it */
+#define MIXER_CFG_NO_WRAP_CHECK 32 /* skip the imaginary -128 test (mixer_stats.wrap128 stays 0):
+                                  for sources that clip to [-127, 127] as the reference's
+                                  transmitter does (src/tx_host.cu:54-55); about 5 % of the
+                                  256 x 128 correlator. No reference counterpart. */
 
 void mixer_default_cfg(mixer_array_cfg* cfg);
 int mixer_validate_cfg(const mixer_array_cfg* cfg);
