@@ -5,7 +5,7 @@
 //              [--dump-mode cumulative|integ] [--elements N] [--grid-x G] [--channels C]
 //              [--k-update K] [--ng NG] [--rate GBps] [--sigma S] [--seed X] [--shm NAME]
 //              [--rho R] [--torbit S] [--refresh-hz F] [--eth-dst MAC] [--frame dv|latest]
-//              [--slots N]
+//              [--slots N] [--no-wrap-check]
 // --rho/--torbit/--refresh-hz move the synthetic planet on the orbit (l, m) = rho (cos, sin)
 // (2 pi t / torbit), re-generating one replay slot per refresh (src/tx_host.cu:84-86,116-125);
 // --eth-dst sets the wire header's destination MAC (src/tx_host.cu:87,96-98).
@@ -13,6 +13,8 @@
 // as the reference's monitor does (dV, src/rx_host_corr.cu:218-222); --frame latest publishes the
 // latest integration's image. --slots N bounds the ingest slots (default 96, the reference's
 // NSLOT, src/rx_host_corr.cu:46): a time block that finds none free is dropped (imissed).
+// --no-wrap-check skips the imaginary -128 test (the synthetic source clips to [-127, 127], as
+// the reference's transmitter does, src/tx_host.cu:54-55).
 //
 // The first positional argument (the reference's DAQIRI YAML) is accepted and ignored.
 // --flush N sets the samples per integration to N V-updates (the reference flushes its int32
@@ -60,7 +62,7 @@ int main(int argc, char** argv) {
     cfg.k_update = 1024;  // KPCH (src/rx_host_corr.cu:40)
     int seconds = 0, device = 0, flush = 8;
     float fps = 20.f;
-    bool noflop = false, dump_cum = true, frame_dv = true;
+    bool noflop = false, dump_cum = true, frame_dv = true, no_wrap = false;
     int slots = 96;
     const char* dump_path = nullptr;
     const char* shm_name = "/corr32";
@@ -95,6 +97,7 @@ int main(int argc, char** argv) {
         else if (a == "--refresh-hz") refresh_hz = atof(nxt());
         else if (a == "--frame") frame_dv = std::string(nxt()) != "latest";
         else if (a == "--slots") slots = atoi(nxt());
+        else if (a == "--no-wrap-check") no_wrap = true;
         else if (a == "--eth-dst") {
             const char* s = nxt();
             unsigned m[6];
@@ -108,7 +111,8 @@ int main(int argc, char** argv) {
             fprintf(stderr, "usage: %s [cfg] [--seconds N][--fps F][--device D][--noflop][--flush N][--dump PATH]\n"
                             "  [--dump-mode cumulative|integ][--elements N][--grid-x G][--channels C][--k-update K]\n"
                             "  [--ng NG][--rate GBps][--sigma S][--seed X][--shm NAME]\n"
-                            "  [--rho R][--torbit S][--refresh-hz F][--eth-dst MAC][--frame dv|latest][--slots N]\n",
+                            "  [--rho R][--torbit S][--refresh-hz F][--eth-dst MAC][--frame dv|latest][--slots N]\n"
+                            "  [--no-wrap-check]\n",
                     argv[0]);
             return 1;
         } else if (a[0] == '-' && a != "-") { fprintf(stderr, "unknown flag %s\n", a.c_str()); return 1; }
@@ -119,6 +123,7 @@ int main(int argc, char** argv) {
     if (dump_path && dump_cum) cfg.flags |= MIXER_CFG_CUMULATIVE;
     if (noflop) cfg.flags |= MIXER_CFG_NOFLOP;
     if (frame_dv) cfg.flags |= MIXER_CFG_FRAME_DV;
+    if (no_wrap) cfg.flags |= MIXER_CFG_NO_WRAP_CHECK;
     if (slots < 0) { fprintf(stderr, "--slots must be >= 0\n"); return 1; }
     CK(mixer_validate_cfg(&cfg));
     std::signal(SIGINT, on_sig);

@@ -85,7 +85,8 @@ def test_cli_orbiting_planet_moves_along_the_orbit():
     shm = f"/mixer_cli_orbit_{os.getpid()}"
     cmd = [BIN, "--seconds", "6", "--elements", str(N), "--grid-x", "16", "--channels", str(C),
            "--k-update", "1024", "--flush", "2", "--ng", "32", "--rate", "0.5", "--fps", "5",
-           "--rho", str(rho), "--torbit", str(torbit), "--refresh-hz", "100", "--shm", shm]
+           "--rho", str(rho), "--torbit", str(torbit), "--refresh-hz", "100", "--shm", shm,
+           "--no-wrap-check"]  # the synthetic source clips to [-127, 127]
     print(" ".join(cmd), flush=True)
     p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     pos = {}
