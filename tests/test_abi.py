@@ -31,6 +31,15 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == bound, set(names) ^ bound
 
 
+def test_binding_constants_match_the_header():
+    """Status codes (enum) and config flags (#define) of the binding are the header's."""
+    src = open(os.path.join(ROOT, "include", "mixer.h")).read()
+    flags = {k: int(v) for k, v in re.findall(r"#define\s+MIXER_(CFG_[A-Z0-9_]+)\s+(\d+)", src)}
+    assert len(flags) >= 6 and all(getattr(M, k) == v for k, v in flags.items()), flags
+    status = {k: int(v) for k, v in re.findall(r"^\s*MIXER_(OK|ERR_[A-Z0-9_]+)\s*=\s*(-?\d+)", src, flags=re.M)}
+    assert len(status) >= 8 and all(getattr(M, k) == v for k, v in status.items()), status
+
+
 def test_default_cfg_is_reference_constants():
     c = M.default_cfg()
     # src/config.h:26-38 and src/rx_host_corr.cu:47,51
