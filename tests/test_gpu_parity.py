@@ -486,6 +486,39 @@ def test_autotune_sets_a_valid_plan():
         M.lib().mixer_set_tuning(128, M.C.byref(M.Tuning(0, 0, 0, 0, 0)))
 
 
+def test_autotune_blocked_then_context_is_exact():
+    """mixer_autotune on a blocked size (512 elements x 64 channels, one launch per integration,
+    enough workgroups for one time slot) also decides between the integrate in the correlator's
+    epilogue and the separate kernel when a one-slot plan wins; a context created afterwards runs
+    the chosen plan and its V and images match the oracle (channels 0, 31 and 63)."""
+    N, C, n_tb = 512, 64, 32
+    cfg = cfg_for(N, C, 32 * n_tb)
+    try:
+        best = M.autotune(cfg)
+        log(f"autotune {N} x {C} x {32 * n_tb} -> {best.as_dict()}")
+        assert best.mfma == 32
+        # the choice exists only for one-slot plans staged from the heaps two blocks ahead
+        one_launch = best.slots == 1 and best.staging == 1 and best.prefetch == 3
+        assert best.integrate in ((1, 2) if one_launch else (0,))
+        bb = rand_heaps(n_tb, N, C)
+        d = M.DeviceBuffer.from_array(bb)
+        with M.Engine(cfg) as e:
+            e.submit_device(d, n_tb, token=False)
+            e.sync()
+            v, idx = e.dump()
+            img, _, _, _ = e.read_image()
+        assert idx == 1
+        for c in (0, 31, 63):
+            ore, oim, of = O.xengine(bb, n_tb, N, C, 32, c, c + 1, nthreads=min(16, os.cpu_count() or 1))
+            assert of == 0
+            assert np.array_equal(v[c], np.stack([ore[0], oim[0]]).astype(np.float32)), f"channel {c}"
+            oimg, _, _, _, _ = O.image_from_v(ore.astype(np.int64), oim.astype(np.int64), cfg.grid_x, cfg.ng)
+            err = np.abs(img[c].astype(np.float64) - oimg[0]).max() / np.abs(oimg[0]).max()
+            assert err <= 1e-5, f"channel {c} image error {err:.2e}"
+    finally:
+        M.set_tuning(N)
+
+
 # ---------------------------------------------------------------- fused corner-turn + correlate
 @pytest.mark.parametrize("N,C,n_tb,parts", [
     (256, 5, 8, 1), (256, 3, 12, 3), (128, 6, 4, 1), (128, 9, 6, 2), (32, 3, 3, 1), (64, 9, 4, 1),
