@@ -418,10 +418,11 @@ def main():
     cfg_nw.flags |= M.CFG_NO_WRAP_CHECK
     with M.Engine(cfg_nw) as eng_nw:
         s_nw = L.mixer_compute_stream(eng_nw.h)
-        w0 = time.perf_counter()
+        w0, n_warm_nw = time.perf_counter(), 0
         while time.perf_counter() - w0 < args.warmup_s:
             for i in range(16):
                 eng_nw.submit_device(rings[i % 2], n_tb, token=False)
+            n_warm_nw += 16
             eng_nw.sync()
         M.check(L.mixer_event_record(ev0, s_nw))
         for i in range(args.steps):
@@ -431,7 +432,7 @@ def main():
         M.check(L.mixer_event_elapsed_ms(ev0, ev1, M.C.byref(ms)))
     no_wrap = {"ms_per_step": round(ms.value / args.steps, 5),
                "value": round(alg["payload_bytes"] * args.steps / (ms.value * 1e-3) / 1e9, 3), "unit": "GB/s",
-               "flags": "MIXER_CFG_NO_WRAP_CHECK"}
+               "flags": "MIXER_CFG_NO_WRAP_CHECK", "warmup_steps_run": n_warm_nw}
     log(f"[bench] without the imaginary -128 test: {no_wrap['ms_per_step']:.4f} ms/step, {no_wrap['value']:.1f} GB/s")
 
     # ---- latency: integrations arriving at 10x the 400 GbE-equivalent rate ----
